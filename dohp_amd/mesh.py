@@ -1,0 +1,65 @@
+"""Synthetic hexahedral block meshes (vertex coordinates only).
+
+Follows the block generator of dohp (src/fs/mesh/interface/genblock.c:145-250):
+vertices on an (m, n, p) lattice over a box, vertex (i, j, k) stored at
+index (i*n + j)*p + k, hexahedra in (i, j, k) order with k fastest.  Optional
+seeded perturbation of interior vertices and the ellip "morph" map
+(src/fs/tests/ellip.c:173-181) produce non-affine elements.
+
+The element->node maps and the rest of the function space are built natively
+by the library (dFSCreateBlock); this module only produces coordinates.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64_uniform(seed: int, count: int, offset: int = 0) -> np.ndarray:
+    """Uniform doubles in [0, 1) from the splitmix64 stream of `seed` (entries offset..offset+count)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def block_vertices(M, box=(-1.0, 1.0, -1.0, 1.0, -1.0, 1.0), perturb: float = 0.0, seed: int = 0,
+                   morph: float = 0.0, twist: float = 0.0, stretch: float = 0.0) -> np.ndarray:
+    """Vertex coordinates [(M0+1)(M1+1)(M2+1), 3] of an M0 x M1 x M2 hexahedral block.
+
+    perturb: interior vertices move by U(-perturb*h, +perturb*h) per coordinate (seeded).
+    morph/twist/stretch: the deformation of ellip.c:173-181 applied after perturbation.
+    """
+    M = tuple(int(v) for v in M)
+    nv = tuple(m + 1 for m in M)
+    axes = [np.linspace(box[2 * d], box[2 * d + 1], nv[d]) for d in range(3)]
+    X = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3).copy()
+    if perturb:
+        h = np.array([(box[2 * d + 1] - box[2 * d]) / M[d] for d in range(3)])
+        I, J, K = np.meshgrid(*[np.arange(n) for n in nv], indexing="ij")
+        interior = ((I > 0) & (I < M[0]) & (J > 0) & (J < M[1]) & (K > 0) & (K < M[2])).reshape(-1)
+        r = splitmix64_uniform(seed, X.size).reshape(-1, 3)
+        X[interior] += (2.0 * r[interior] - 1.0) * perturb * h
+    if morph or twist or stretch:
+        x, y, z = X[:, 0].copy(), X[:, 1].copy(), X[:, 2].copy()
+        X[:, 0] = np.cos(twist * z) * x + np.sin(twist * z) * y
+        X[:, 1] = -np.sin(twist * z) * x + np.cos(twist * z) * y
+        X[:, 2] = -1 + (z + 1) * (1 + morph * ((x + 1) ** 2 + (y + 1) ** 2)) + stretch * z
+    return X
+
+
+# Boundary sets of the block generator (genblock.c:406-420): 100 = the four walls
+# (x and y faces), 200 = top (z max), 300 = bottom (z min).
+DIRICHLET_WALLS, DIRICHLET_TOP, DIRICHLET_BOTTOM = 1, 2, 4
+DIRICHLET_ALL = 7
+
+
+def dirichlet_mask_from_sets(sets) -> int:
+    mask = 0
+    for s in sets:
+        mask |= {100: DIRICHLET_WALLS, 200: DIRICHLET_TOP, 300: DIRICHLET_BOTTOM}[int(s)]
+    return mask

@@ -20,6 +20,13 @@
 #define ORC_PI 3.14159265358979323846264338327950288
 #define ORC_MAXN 16
 
+/* When nonzero, the Q1 assembly reads the inverse Jacobian in patch-natural order (the
+ * documented meaning of dEFSGetExplicitSparse's cjinv) instead of the element-natural
+ * array the reference iterator actually passes.  Tests use it to show that the golden
+ * refouts distinguish the two. */
+static int orc_q1_cjinv_patch_order = 0;
+void orc_set_q1_cjinv_patch_order(int flag) { orc_q1_cjinv_patch_order = flag; }
+
 /* ------------------------------------------------------------------------- */
 /* Jacobi polynomials (polylib.c:800-940 jacobfd/jacobd): three-term recurrence,
  * derivative d/dz P^{a,b}_n = (n+a+b+1)/2 P^{a+1,b+1}_{n-1}.                  */
@@ -809,7 +816,17 @@ long orc_ellip_q1_assemble(const orc_problem *pb, const double *uc, int qmethod,
     }
     int P;
     /* The solution link receives the element-natural cjinv (fsrulesetit.c:684). */
-    if (orc_explicit_sparse_hex(n, r.Q1d, r.rc, npatch, Q, r.patchind, wk.cjinv, 0, &P, eidx, interp, deriv)) { ret = -1; break; }
+    const double *cj = wk.cjinv;
+    double *pcj = NULL;
+    if (orc_q1_cjinv_patch_order && !identity) {
+      pcj = malloc(sizeof(double) * r.nq * 9);
+      for (int i = 0; i < r.nq; i++)
+        for (int l = 0; l < 9; l++) pcj[i * 9 + l] = wk.cjinv[r.patchind[i] * 9 + l];
+      cj = pcj;
+    }
+    const int xe = orc_explicit_sparse_hex(n, r.Q1d, r.rc, npatch, Q, r.patchind, cj, 0, &P, eidx, interp, deriv);
+    free(pcj);
+    if (xe) { ret = -1; break; }
     for (int pt = 0; pt < npatch; pt++) {
       const double *I = &interp[(long)pt * Q * P], *Dd = &deriv[(long)pt * Q * P * 3];
       const int *rc = &eidx[pt * P];

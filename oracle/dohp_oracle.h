@@ -72,6 +72,7 @@ int orc_ellip_jacobian_apply(const orc_problem *pb, const double *uc, const doub
  * written, or -needed if cap is too small.  qmethod SPARSE gives the reference default. */
 long orc_ellip_q1_assemble(const orc_problem *pb, const double *uc, int qmethod, long cap,
                            int *rows, int *cols, double *vals);
+void orc_set_q1_cjinv_patch_order(int flag);
 /* Error norms (util.c:110-168 via ellip.c:538-564). */
 int orc_ellip_error_norms(const orc_problem *pb, const double *uc, double enorm[3], double gnorm[3]);
 

@@ -1,0 +1,192 @@
+"""Benchmark: matrix-free order-6 operator apply (GDOF/s) on one MI355X.
+
+Workload (default): 64^3 hexahedra on [0,1]^3 with interior vertices perturbed by +-10% of h
+(seeded), order p=6 (n=7), homogeneous Dirichlet on all faces for the direction vector,
+nonlinear diffusion -div(k(|grad u|) grad u) with k = (1e-2 + |grad u|^2)^(-1/4), linearized at
+u0 = a seeded sum of 8 random-phase sine modes of amplitude 0.1.  ellip's p-Laplacian with
+p = 1.5, eps^2 = 0.005 has eta = 2^(1/4) k, so the operator is ellip's up to that constant.
+
+A step is one Jacobian-vector product y = J(u0) v (the MATSHELL MatMult a Krylov solver calls)
+in FP64 with v ~ U[-1,1] (seeded), inputs resident in HBM.  value = N_int / t_step in GDOF/s.
+The residual F(u0), the FP32 Jacobian action and the linear (Poisson) apply are timed too and
+reported under "variants".
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def sine_state(X, seed):
+    from dohp_amd.mesh import splitmix64_uniform
+    r = splitmix64_uniform(seed, 32)
+    u = np.zeros(len(X))
+    for m in range(8):
+        k = 1 + np.floor(r[4 * m:4 * m + 3] * 4)
+        u += 0.1 * np.sin(2 * np.pi * (X @ k) + 2 * np.pi * r[4 * m + 3])
+    return u
+
+
+def alg_bytes(s, n_tot, n_int, n, nelem, nvert, jv):
+    """Minimum unique HBM traffic of one apply (geometry recomputed from Q1 vertices)."""
+    b = s * n_tot + s * n_int + 4 * n ** 3 * nelem + 24 * nvert + 32 * nelem
+    return b + (s * n_tot if jv else 0)
+
+
+def time_fn(torch, fn, reps, warm=2):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        fn()
+    t1.record()
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / 1e3 / reps
+
+
+def cpu_baseline(nthreads):
+    """Oracle J(u) v on a bounded sample of the same workload: 12^3 elements, p=6."""
+    from dohp_amd.mesh import block_vertices, splitmix64_uniform
+    from oracle import oracle as O
+    M = (20, 20, 20)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1)
+    S = O.BlockSpace(M, 6, V)
+    E = O.Ellip(S, p=1.5, eps=np.sqrt(0.005), nthreads=nthreads)
+    uc = sine_state(S.nodal_coordinates(), 3)
+    v = np.concatenate([2 * splitmix64_uniform(9, S.nowned) - 1, np.zeros(S.nclosure - S.nowned)])
+    E.jacobian_apply(uc, v)
+    reps, t = 0, 0.0
+    while t < 10.0 and reps < 20:
+        t0 = time.perf_counter()
+        E.jacobian_apply(uc, v)
+        t += time.perf_counter() - t0
+        reps += 1
+    return S.nowned / (t / reps) / 1e9, f"20^3 elements p=6 J(u)v, mean of {reps} applies"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--elems", type=int, default=64)
+    ap.add_argument("--degree", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scatter", type=int, default=0, help="0 = eight-colour deterministic, 1 = atomics")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; this build has no multi-GPU path (see DESIGN.md, sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dohp_amd.capi as C
+    from dohp_amd.mesh import block_vertices, splitmix64_uniform
+
+    dev = torch.device("cuda:0")
+    Me, p = args.elems, args.degree
+    M = (Me, Me, Me)
+    n = p + 1
+    t_setup = time.perf_counter()
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1)
+    fs = C.FS(M, p, V)
+    X = fs.nodal_coordinates()
+    u0 = sine_state(X, 3)
+    del X
+    fs.dirichlet_commit(u0[fs.nowned:])
+    op = C.Ellip(fs, p=1.5, eps=np.sqrt(0.005))
+    op.set_scatter(args.scatter)
+    lin = C.Ellip(fs, p=2.0)
+    lin.set_scatter(args.scatter)
+    u = torch.as_tensor(u0[:fs.nowned], dtype=torch.float64, device=dev)
+    v = torch.as_tensor(2 * splitmix64_uniform(9, fs.nowned) - 1, dtype=torch.float64, device=dev)
+    v32 = v.float()
+    f = torch.zeros_like(u)
+    y = torch.zeros_like(u)
+    y32 = torch.zeros_like(v32)
+    stream = torch.cuda.current_stream()
+    op.function(u, f, stream)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    print(f"# setup {t_setup:.1f}s: {fs.nelem} elements p={p}, N_int={fs.nowned}, N_tot={fs.nclosure}",
+          file=sys.stderr, flush=True)
+
+    # timed region: exactly `steps` FP64 Jacobian actions, synchronized on both sides
+    for _ in range(args.warmup):
+        op.mat_mult(v, y, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        op.mat_mult(v, y, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_step = e0.elapsed_time(e1) / 1e3 / args.steps
+    print(f"# Jv fp64 {t_step * 1e6:.1f} us", file=sys.stderr, flush=True)
+
+    variants = {}
+    reps = max(3, args.steps // 2)
+    for name, fn, s, jv in [("residual_f64", lambda: op.function(u, f, stream), 8, True),
+                            ("jv_f32", lambda: op.mat_mult_f32(v32, y32, stream), 4, True),
+                            ("linear_f64", lambda: lin.mat_mult(v, y, stream), 8, False),
+                            ("linear_f32", lambda: lin.mat_mult_f32(v32, y32, stream), 4, False)]:
+        t = time_fn(torch, fn, reps)
+        b = alg_bytes(s, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, jv)
+        variants[name] = {"us": round(t * 1e6, 1), "gdofs": round(fs.nowned / t / 1e9, 2),
+                          "hbm_frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}
+        print(f"# {name} {t * 1e6:.1f} us", file=sys.stderr, flush=True)
+    # FP32 error against the FP64 apply (same inputs)
+    op.mat_mult(v, y, stream)
+    op.mat_mult_f32(v32, y32, stream)
+    torch.cuda.synchronize()
+    variants["jv_f32"]["rel_err_vs_f64"] = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
+
+    B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True)
+    achieved = B / t_step / 1e9
+    cpu = None
+    if not args.no_cpu:
+        ncores = os.cpu_count() or 1
+        try:
+            ncores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            pass
+        val, sample = cpu_baseline(ncores)
+        cpu = {"value": round(val, 4), "unit": "GDOF/s", "cores": ncores, "kind": "port", "sample": sample}
+    out = {
+        "metric": "GDOF/s operator apply at order 6",
+        "value": round(fs.nowned / t_step / 1e9, 3),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"nonlinear diffusion J(u0)v, {Me}^3 hex, p={p}, FP64, perturbed mesh",
+                   "elements": fs.nelem, "degree": p, "n_int": fs.nowned, "n_tot": fs.nclosure,
+                   "scatter": "colored" if args.scatter == 0 else "atomic"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,8 @@
+"""dohp_amd -- MI355X-native matrix-free hp element operator of dohp (jedbrown/dohp).
+
+Submodules:
+  capi      ctypes binding of the HIP library's C ABI (include/dohp_amd.h); raises on
+            import when dohp_amd/libdohp_amd.so has not been built.
+  mesh      synthetic hexahedral block meshes (vertex coordinates).
+  solvers   host drivers above the C ABI (Krylov, Newton, defect correction).
+"""

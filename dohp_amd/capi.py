@@ -1,0 +1,212 @@
+"""ctypes binding of the dohp_amd HIP library (host-side mirror of the C ABI).
+
+The compute path is the HIP library dohp_amd/libdohp_amd.so (built from
+dohp_amd/csrc by __graft_entry__.build()).  This module only binds its C ABI
+(include/dohp_amd.h) with ctypes and offers thin objects shaped like dohp's
+dFS / dEFS / MATSHELL interfaces.  Device arrays are passed as raw pointers;
+tests and benchmarks allocate them with torch.  There is no CPU fallback: if the
+library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libdohp_amd.so")
+
+APPLY_INTERP, APPLY_INTERP_TRANSPOSE, APPLY_GRAD, APPLY_GRAD_TRANSPOSE = 40001, 40002, 40003, 40004
+INSERT_VALUES, ADD_VALUES = 1, 2
+HOMOGENEOUS, INHOMOGENEOUS = 0, 1
+QUAD_FAST, QUAD_SPARSE, QUAD_SELF = 0, 1, 2
+GAUSS_LEGENDRE, GAUSS_LOBATTO = 0, 1
+ERR_NEGATIVE_JACOBIAN = 77
+
+if not os.path.exists(LIBPATH):
+    raise ImportError(f"dohp_amd: HIP library not built ({LIBPATH}); run __graft_entry__.build()")
+_lib = ctypes.CDLL(LIBPATH)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_long = ctypes.c_long
+
+
+class EllipParam(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_double), ("epsilon", ctypes.c_double), ("lam", ctypes.c_double),
+                ("exact", ctypes.c_int), ("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double)]
+
+
+def _proto(name, *args):
+    f = getattr(_lib, name)
+    f.argtypes = list(args)
+    f.restype = ctypes.c_int
+    return f
+
+
+_proto("dohpTensorRule", ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp)
+_proto("dohpTensorBasis", ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp)
+_proto("dEFSApplyHexBatched", ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp,
+       ctypes.c_int, ctypes.c_int, _vp)
+_proto("dFSCreateBlock", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(_vp))
+_proto("dFSDestroy", ctypes.POINTER(_vp))
+_proto("dFSGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long), ctypes.POINTER(_long), _ip)
+_proto("dFSGetNodalCoordinatesClosure", _vp, _dp)
+_proto("dFSInhomogeneousDirichletCommit", _vp, _dp)
+_proto("dFSGlobalToExpanded", _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp)
+_proto("dFSExpandedToGlobal", _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp)
+_proto("dFSGetElementNodes", _vp, _ip)
+_proto("dohpEllipCreate", _vp, ctypes.POINTER(EllipParam), ctypes.POINTER(_vp))
+_proto("dohpEllipDestroy", ctypes.POINTER(_vp))
+_proto("dohpEllipSetCoefficient", _vp, _dp)
+_proto("dohpEllipFunction", _vp, _vp, _vp, _vp)
+_proto("dohpEllipShellMatMult", _vp, _vp, _vp, _vp)
+_proto("dohpEllipShellMatMultF32", _vp, _vp, _vp, _vp)
+_proto("dohpEllipSetScatter", _vp, ctypes.c_int)
+_lib.dohpGetLastError.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+_lib.dohpGetLastError.restype = None
+
+
+class DohpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dohp_amd error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc:
+        buf = ctypes.create_string_buffer(512)
+        _lib.dohpGetLastError(buf, 512)
+        raise DohpError(rc, buf.value.decode())
+    return rc
+
+
+def lib():
+    return _lib
+
+
+def _ptr(a):
+    """Device or host pointer of a torch tensor (or None)."""
+    if a is None:
+        return None
+    return ctypes.c_void_p(a.data_ptr())
+
+
+def _np(a, dtype=np.float64):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(_dp if dtype == np.float64 else _ip)
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+# ----------------------------------------------------------------- dJacobi
+def tensor_rule(method, family, order):
+    size = ctypes.c_int()
+    z, w = np.zeros(order + 4), np.zeros(order + 4)
+    check(_lib.dohpTensorRule(method, family, order, ctypes.byref(size), z.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
+    return z[:size.value].copy(), w[:size.value].copy()
+
+
+def tensor_basis(P, rcoord):
+    rc, rcp = _np(rcoord)
+    Q = rc.size
+    B, D, node, weight = np.zeros(Q * P), np.zeros(Q * P), np.zeros(P), np.zeros(P)
+    check(_lib.dohpTensorBasis(P, Q, rcp, B.ctypes.data_as(_dp), D.ctypes.data_as(_dp), node.ctypes.data_as(_dp),
+                               weight.ctypes.data_as(_dp)))
+    return B.reshape(Q, P), D.reshape(Q, P), node, weight
+
+
+def efs_apply_hex_batched(P, rcoord, nelem, jinv, D, x, y, amode, imode=INSERT_VALUES, stream=None):
+    """dEFSApply over a batch of elements; jinv/x/y are device tensors (float64)."""
+    rc, rcp = _np(rcoord)
+    check(_lib.dEFSApplyHexBatched(P, rc.size, rcp, nelem, _ptr(jinv), D, _ptr(x), _ptr(y), amode, imode,
+                                   _stream(stream)))
+
+
+# ---------------------------------------------------------------------- dFS
+class FS:
+    """Continuous degree-p space on a structured hexahedral block (dFS)."""
+
+    def __init__(self, M, degree, vcoords, dirichlet_mask=7):
+        self.M = tuple(int(m) for m in M)
+        self.degree = int(degree)
+        v, vp = _np(vcoords)
+        h = _vp()
+        check(_lib.dFSCreateBlock(self.M[0], self.M[1], self.M[2], self.degree, int(dirichlet_mask), vp,
+                                  ctypes.byref(h)))
+        self.h = h
+        a, b, c, d = _long(), _long(), _long(), ctypes.c_int()
+        check(_lib.dFSGetSizes(h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+        self.nowned, self.nclosure, self.nexpanded, self.nelem = a.value, b.value, c.value, d.value
+        self.nvert = v.shape[0] if v.ndim == 2 else v.size // 3
+
+    def nodal_coordinates(self):
+        X = np.zeros((self.nclosure, 3))
+        check(_lib.dFSGetNodalCoordinatesClosure(self.h, X.ctypes.data_as(_dp)))
+        return X
+
+    def element_nodes(self):
+        n = self.degree + 1
+        E = np.zeros((self.nelem, n ** 3), dtype=np.int32)
+        check(_lib.dFSGetElementNodes(self.h, E.ctypes.data_as(_ip)))
+        return E
+
+    def dirichlet_commit(self, dvals):
+        d, dp = _np(dvals)
+        assert d.size == self.nclosure - self.nowned
+        check(_lib.dFSInhomogeneousDirichletCommit(self.h, dp))
+
+    def global_to_expanded(self, g, x, hmode=INHOMOGENEOUS, imode=INSERT_VALUES, stream=None):
+        check(_lib.dFSGlobalToExpanded(self.h, _ptr(g), _ptr(x), hmode, imode, _stream(stream)))
+
+    def expanded_to_global(self, x, g, hmode=INHOMOGENEOUS, imode=INSERT_VALUES, stream=None):
+        check(_lib.dFSExpandedToGlobal(self.h, _ptr(x), _ptr(g), hmode, imode, _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.dFSDestroy(ctypes.byref(self.h))
+            self.h = None
+
+
+# ------------------------------------------------------------------- ellip
+class Ellip:
+    """ellip's residual and MATSHELL Jacobian action on the GPU (src/fs/tests/ellip.c:407-472)."""
+
+    def __init__(self, fs: FS, p=2.0, eps=1.0, lam=0.0, exact=-1, a=1.0, b=1.0, c=1.0, kvert=None):
+        self.fs = fs
+        self.prm = EllipParam(float(p), float(eps), float(lam), int(exact), float(a), float(b), float(c))
+        h = _vp()
+        check(_lib.dohpEllipCreate(fs.h, ctypes.byref(self.prm), ctypes.byref(h)))
+        self.h = h
+        if kvert is not None:
+            self.set_coefficient(kvert)
+
+    def set_coefficient(self, kvert):
+        if kvert is None:
+            check(_lib.dohpEllipSetCoefficient(self.h, None))
+        else:
+            k, kp = _np(kvert)
+            check(_lib.dohpEllipSetCoefficient(self.h, kp))
+
+    def set_scatter(self, mode):
+        check(_lib.dohpEllipSetScatter(self.h, int(mode)))
+
+    def function(self, u, f, stream=None):
+        check(_lib.dohpEllipFunction(self.h, _ptr(u), _ptr(f), _stream(stream)))
+
+    def mat_mult(self, x, y, stream=None):
+        check(_lib.dohpEllipShellMatMult(self.h, _ptr(x), _ptr(y), _stream(stream)))
+
+    def mat_mult_f32(self, x, y, stream=None):
+        check(_lib.dohpEllipShellMatMultF32(self.h, _ptr(x), _ptr(y), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.dohpEllipDestroy(ctypes.byref(self.h))
+            self.h = None

@@ -1,0 +1,98 @@
+// Internal declarations shared by the dohp_amd HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/dohp_amd.h"
+
+namespace dohp {
+
+void set_error(const std::string &msg);
+
+#define DOHP_HIP(call)                                                                              \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess) {                                                                         \
+      ::dohp::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + ":" + \
+                        std::to_string(__LINE__));                                                  \
+      return dohpERR_HIP;                                                                           \
+    }                                                                                               \
+  } while (0)
+
+#define DOHP_CHECK(cond, code, msg)     \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::dohp::set_error(msg);           \
+      return (code);                    \
+    }                                   \
+  } while (0)
+
+#define DOHP_CALL(call)       \
+  do {                        \
+    dErr r_ = (call);         \
+    if (r_) return r_;        \
+  } while (0)
+
+constexpr int kMaxN = 16;
+
+// 1-D rules and bases (host).
+int gauss_legendre(int n, double *z, double *w);
+int gauss_lobatto(int n, double *z, double *w);
+void barycentric_weights(int n, const double *x, double *bw);
+// interp[q][j] = l_j(xq[q]) for the Lagrange basis through x[0..n)
+void lagrange_matrix(int n, const double *x, int m, const double *xq, double *interp);
+// D[i][j] = l_j'(x[i]) for the Lagrange basis through x[0..n)
+void collocation_derivative(int n, const double *x, double *D);
+
+// Device buffer helper.
+template <typename T>
+struct DevBuf {
+  T *ptr = nullptr;
+  size_t n = 0;
+  dErr alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return 0;
+    if (hipMalloc(&ptr, count * sizeof(T)) != hipSuccess) {
+      ptr = nullptr;
+      set_error("hipMalloc failed for " + std::to_string(count * sizeof(T)) + " bytes");
+      return dohpERR_HIP;
+    }
+    return 0;
+  }
+  dErr upload(const T *h, size_t count) {
+    DOHP_CALL(alloc(count));
+    if (count) DOHP_HIP(hipMemcpy(ptr, h, count * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+}  // namespace dohp
+
+// Function space on a structured hexahedral block.
+struct _p_dohpFS {
+  int M[3] = {0, 0, 0};
+  int p = 0, n = 0;
+  int nelem = 0;
+  long nowned = 0, nclosure = 0;
+  int dirichlet_mask = 0;
+  long nvert = 0;
+  std::vector<double> vcoords;       // [nvert][3]
+  std::vector<int> elem_nodes;       // [nelem][n^3]
+  std::vector<int> elem_verts;       // [nelem][8], tensor order
+  std::vector<int> color_off;        // 9 offsets into color_elems
+  std::vector<int> color_elems;      // elements grouped by (ex%2, ey%2, ez%2)
+  dohp::DevBuf<int> d_elem_nodes, d_elem_verts, d_color_elems;
+  dohp::DevBuf<double> d_vcoords, d_dcache;
+  dohp::DevBuf<float> d_dcache32;
+};

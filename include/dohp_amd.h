@@ -1,0 +1,119 @@
+/*
+ * dohp_amd.h -- C ABI of the MI355X matrix-free hp element operator.
+ *
+ * Drop-in boundary for dohp's operator path (jedbrown/dohp).  Every entry point
+ * returns a dErr (0 = success, nonzero = error; dohpGetLastError gives the text),
+ * never aborts, and takes plain pointers and sizes.  Pointers named *_dev are
+ * device (HBM) pointers; *_host are host pointers.  `stream` is a hipStream_t
+ * (NULL = the default stream).  Callers own every array they pass in.
+ *
+ * Layouts follow the reference (Appendix conventions of dohp):
+ *   element node index (i*n + j)*n + k, k fastest;  quadrature index likewise;
+ *   block size D interleaved fastest;  GRAD output [Q][D][3];
+ *   jinv[q][k][e] = dX_k/dx_e;  closure vectors are [owned | Dirichlet].
+ */
+#ifndef DOHP_AMD_H
+#define DOHP_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int dErr;
+
+/* dApplyMode (include/dohpjacobi.h:101-108) and PETSc InsertMode values. */
+enum { dAPPLY_INTERP = 40001, dAPPLY_INTERP_TRANSPOSE, dAPPLY_GRAD, dAPPLY_GRAD_TRANSPOSE };
+enum { dINSERT_VALUES = 1, dADD_VALUES = 2 };
+/* dFSHomogeneousMode (include/dohpfs.h:49). */
+enum { dFS_HOMOGENEOUS = 0, dFS_INHOMOGENEOUS = 1 };
+/* dQuadratureMethod (include/dohpjacobi.h:143-150) and dGaussFamily (:218). */
+enum { dQUADRATURE_METHOD_FAST = 0, dQUADRATURE_METHOD_SPARSE = 1, dQUADRATURE_METHOD_SELF = 2 };
+enum { dGAUSS_LEGENDRE = 0, dGAUSS_LOBATTO = 1 };
+/* Precision of an operator application. */
+enum { dohpPREC_F64 = 0, dohpPREC_F32 = 1 };
+/* Error codes beyond generic failure. */
+enum { dohpERR_GENERIC = 1, dohpERR_ARG = 62, dohpERR_NEGATIVE_JACOBIAN = 77, dohpERR_HIP = 99 };
+
+/* Copies the message of the last error on this thread into buf. */
+void dohpGetLastError(char *buf, size_t len);
+
+/* ------------------------------------------------------------------ dJacobi */
+/* 1-D rule of TensorGetRule (src/jacobi/quadrature/impls/tensor/tensorquad.c:51-118):
+ * method FAST/SPARSE/SELF, family Gauss/Lobatto, polynomial order.  coord/weight need
+ * room for order+4 entries; *size receives the rule size. */
+dErr dohpTensorRule(int method, int family, int order, int *size, double *coord_host, double *weight_host);
+
+/* Nodal GLL basis of TensorBasisCreate (src/jacobi/impls/tensor/tensor.c:22-109):
+ * interp[Q][P] = l_j(rcoord_q), deriv = interp * D_gll, plus GLL nodes and weights. */
+dErr dohpTensorBasis(int P, int Q, const double *rcoord_host, double *interp_host, double *deriv_host,
+                     double *node_host, double *weight_host);
+
+/* --------------------------------------------------------------------- dEFS */
+/* Batched dEFSApply on isotropic hexahedra (replaces dEFSApply, src/jacobi/interface/jacobi.c:489-506,
+ * dispatched to dEFSApply_Tensor_Hex, src/jacobi/impls/tensor/efstopo.c:368-423).
+ * nelem elements, each with in/out slices of the per-mode sizes (INTERP: P^3*D -> Q^3*D,
+ * GRAD: P^3*D -> Q^3*D*3, transposes the reverse), consecutive per element.  jinv_dev is
+ * [nelem][Q^3][9] or NULL (GRAD only: layout swap, efstopo.c:518-546).  rcoord is the 1-D rule. */
+dErr dEFSApplyHexBatched(int P, int Q, const double *rcoord_host, int nelem, const double *jinv_dev, int D,
+                         const double *in_dev, double *out_dev, int amode, int imode, void *stream);
+
+/* ---------------------------------------------------------------------- dFS */
+typedef struct _p_dohpFS *dohpFS;
+
+/* Continuous degree-p space on an M0 x M1 x M2 hexahedral block (replaces dFSCreate +
+ * dFSSetMesh/dFSSetDegree/dFSRegisterBoundary/dFSBuildSpace, src/fs/interface/fs.c:11-110,423,
+ * src/fs/impls/cont/cont.c:258-385).  vcoords_host: [(M0+1)(M1+1)(M2+1)][3] vertex coordinates in
+ * block-generator order (src/fs/mesh/interface/genblock.c:206-227).  dirichlet_mask: bit 1 = walls
+ * (set 100), 2 = top (200), 4 = bottom (300).  Closure numbering: owned nodes first, then Dirichlet. */
+dErr dFSCreateBlock(int M0, int M1, int M2, int degree, int dirichlet_mask, const double *vcoords_host,
+                    dohpFS *fs);
+dErr dFSDestroy(dohpFS *fs);
+/* Sizes: owned (global) nodes, closure nodes, expanded nodes (nelem * n^3), elements. */
+dErr dFSGetSizes(dohpFS fs, long *nowned, long *nclosure, long *nexpanded, int *nelem);
+/* Closure nodal coordinates averaged over elements (src/fs/interface/fsgeom.c:72-109), [nclosure][3]. */
+dErr dFSGetNodalCoordinatesClosure(dohpFS fs, double *coords_host);
+/* Cache Dirichlet values (replaces dFSInhomogeneousDirichletCommit, src/fs/interface/fs.c:523-537);
+ * dvals_host holds the (nclosure - nowned) Dirichlet entries of the closure. */
+dErr dFSInhomogeneousDirichletCommit(dohpFS fs, const double *dvals_host);
+/* Gather/scatter of src/fs/interface/fs.c:543-640 on device vectors (FP64).  g is the global
+ * (owned) vector, x the expanded vector [nelem][n^3]. */
+dErr dFSGlobalToExpanded(dohpFS fs, const double *g_dev, double *x_dev, int hmode, int imode, void *stream);
+dErr dFSExpandedToGlobal(dohpFS fs, const double *x_dev, double *g_dev, int hmode, int imode, void *stream);
+/* Element -> closure node map [nelem][n^3] (the E matrix, src/fs/impls/cont/cont.c:200-249). */
+dErr dFSGetElementNodes(dohpFS fs, int *elem_nodes_host);
+
+/* ------------------------------------------------------------ ellip operator */
+typedef struct {
+  double p;        /* p of the p-Laplacian (ellip.c:27-32, -ellip_p) */
+  double epsilon;  /* regularization (-ellip_eps) */
+  double lambda;   /* Bratu nonlinearity (-ellip_lam) */
+  int exact;       /* forcing of exact solution 0..4 (ellip.c:42-117, -exact); -1 = f = 0 */
+  double a, b, c;  /* exact-solution parameters (-exact_a/b/c) */
+} dohpEllipParam;
+
+typedef struct _p_dohpEllip *dohpEllip;
+
+/* Operator context: the MATSHELL context of ellip (src/fs/tests/ellip.c:697-699).  Validates the
+ * geometry: a nonpositive Jacobian determinant at any quadrature point returns
+ * dohpERR_NEGATIVE_JACOBIAN (src/jacobi/interface/jacobi.c:397). */
+dErr dohpEllipCreate(dohpFS fs, const dohpEllipParam *prm, dohpEllip *op);
+dErr dohpEllipDestroy(dohpEllip *op);
+/* Vertex coefficient field k (length = number of vertices, genblock order), multiplying the
+ * diffusion: -div(k eta grad u).  NULL restores k = 1. */
+dErr dohpEllipSetCoefficient(dohpEllip op, const double *kvert_host);
+/* Residual F(u) (EllipFunction, src/fs/tests/ellip.c:407-440) with inhomogeneous Dirichlet values;
+ * records u as the linearization state of later Jacobian applications (the store of ellip.c:311). */
+dErr dohpEllipFunction(dohpEllip op, const double *u_dev, double *f_dev, void *stream);
+/* y = J(u) x, homogeneous Dirichlet (EllipShellMatMult, src/fs/tests/ellip.c:442-472): the MATSHELL
+ * MATOP_MULT body.  FP64 and FP32 variants; the FP32 one keeps geometry in FP64. */
+dErr dohpEllipShellMatMult(dohpEllip op, const double *x_dev, double *y_dev, void *stream);
+dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x_dev, float *y_dev, void *stream);
+/* Scatter strategy of the element kernels: 0 = eight-colour deterministic (default), 1 = atomics. */
+dErr dohpEllipSetScatter(dohpEllip op, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle, on the same seeded inputs.
+
+FP64 bar: relative 2-norm difference <= 1e-12.  FP32 results state their error against
+the FP64 oracle.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from dohp_amd.mesh import block_vertices, splitmix64_uniform  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+G = os.path.join(os.path.dirname(__file__), "golden")
+TOL64 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def C():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import dohp_amd.capi as capi
+    return capi
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def setup(C, M, deg, V, mask=7, **prm):
+    fs = C.FS(M, deg, V, dirichlet_mask=mask)
+    S = O.BlockSpace(M, deg, V, dirichlet_mask=mask)
+    gexact = prm.pop("gexact", -1)
+    E = O.Ellip(S, exact=gexact, **prm)
+    op = C.Ellip(fs, exact=gexact, **prm)
+    return fs, S, E, op
+
+
+# ---------------------------------------------------------------- dEFSApply
+@pytest.mark.parametrize("P,Q,D", [(2, 2, 3), (3, 4, 1), (5, 5, 1), (7, 7, 1), (4, 6, 2), (11, 11, 1)])
+def test_efs_apply_batched_all_modes(C, P, Q, D):
+    print(f"efs P={P} Q={Q} D={D}", flush=True)
+    rng = np.random.default_rng(P * 31 + Q * 7 + D)
+    ne = 5
+    rc, _ = O.gauss_legendre(Q)
+    jinv = rng.uniform(-1, 1, (ne, Q ** 3, 9))
+    sizes = {C.APPLY_INTERP: (P ** 3 * D, Q ** 3 * D), C.APPLY_GRAD: (P ** 3 * D, Q ** 3 * D * 3),
+             C.APPLY_INTERP_TRANSPOSE: (Q ** 3 * D, P ** 3 * D), C.APPLY_GRAD_TRANSPOSE: (Q ** 3 * D * 3, P ** 3 * D)}
+    for amode, (nin, nout) in sizes.items():
+        for imode in (C.INSERT_VALUES, C.ADD_VALUES):
+            for use_j in ((True, False) if amode == C.APPLY_GRAD else (True,)):
+                x = rng.uniform(-1, 1, (ne, nin))
+                y0 = rng.uniform(-1, 1, (ne, nout))
+                yd = dev(y0)
+                jd = dev(jinv) if use_j else None
+                C.efs_apply_hex_batched(P, rc, ne, jd, D, dev(x), yd, amode, imode)
+                got = yd.cpu().numpy()
+                for e in range(ne):
+                    ref = O.efs_apply_hex(P, rc, jinv[e] if use_j else None, D, x[e], amode, imode, out=y0[e])
+                    assert rel(got[e], ref) < 1e-13, (amode, imode, e)
+
+
+# ------------------------------------------------------------ gather/scatter
+def test_gather_scatter(C):
+    M, p = (3, 4, 2), 3
+    V = block_vertices(M, perturb=0.1, seed=5)
+    fs = C.FS(M, p, V)
+    S = O.BlockSpace(M, p, V)
+    rng = np.random.default_rng(0)
+    g = rng.uniform(-1, 1, fs.nowned)
+    dv = rng.uniform(-1, 1, fs.nclosure - fs.nowned)
+    fs.dirichlet_commit(dv)
+    x = torch.zeros(fs.nexpanded, dtype=torch.float64, device="cuda")
+    fs.global_to_expanded(dev(g), x, C.INHOMOGENEOUS)
+    np.testing.assert_array_equal(x.cpu().numpy(), np.concatenate([g, dv])[S.elem_nodes.reshape(-1)])
+    fs.global_to_expanded(dev(g), x, C.HOMOGENEOUS)
+    np.testing.assert_array_equal(x.cpu().numpy(), np.concatenate([g, 0 * dv])[S.elem_nodes.reshape(-1)])
+    xe = rng.uniform(-1, 1, fs.nexpanded)
+    gd = dev(np.ones(fs.nowned))
+    fs.expanded_to_global(dev(xe), gd, C.INHOMOGENEOUS, C.ADD_VALUES)
+    ref = np.ones(fs.nclosure)
+    np.add.at(ref, S.elem_nodes.reshape(-1), xe)
+    np.testing.assert_allclose(gd.cpu().numpy(), ref[:fs.nowned], rtol=1e-15, atol=1e-15)
+
+
+# ----------------------------------------- config 1: 4^3 p=4 Poisson, perturbed
+def test_config1_poisson_apply_and_residual(C):
+    print("config1 4^3 p=4 perturbed", flush=True)
+    M, p = (4, 4, 4), 4
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1)
+    fs, S, E, op = setup(C, M, p, V, p=2.0, eps=1.0, gexact=4)
+    x = 2 * splitmix64_uniform(7, fs.nowned) - 1
+    dv = E.dirichlet_values()
+    fs.dirichlet_commit(dv)
+    y = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    op.mat_mult(dev(x), y)
+    uc0 = E.closure(np.zeros(fs.nowned), np.zeros(len(dv)))
+    ref = E.jacobian_apply(uc0, np.concatenate([x, np.zeros(len(dv))]))[:fs.nowned]
+    assert rel(y.cpu().numpy(), ref) < TOL64
+    f = torch.zeros_like(y)
+    op.function(dev(x), f)
+    reff = E.function(E.closure(x, dv))[:fs.nowned]
+    assert rel(f.cpu().numpy(), reff) < TOL64
+
+
+# --------------------------------------------------------- ellip refout norms
+HIST = json.load(open(os.path.join(G, "ellip_histories.json")))
+CASES = {"ellip-e0-b1-p16": (1, 1.6, 0.0, (1.6, 1.7, 1.8)), "ellip-e0-b3-p16": (3, 1.6, 0.0, (1.6, 1.7, 1.8)),
+         "ellip-e0-b3-p14-sse": (3, 1.4, 0.0, (1.6, 1.7, 1.8)), "ellip-e0-b3-p17-lam1-sse": (3, 1.7, 1.0, (1, 1, 1))}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_ellip_refout_initial_residual_on_gpu(C, name):
+    print(name, flush=True)
+    bdeg, ep, lam, (a, b, c) = CASES[name]
+    V = block_vertices((4, 4, 4))
+    fs, S, E, op = setup(C, (4, 4, 4), bdeg, V, p=ep, eps=1.0, lam=lam, gexact=0, a=a, b=b, c=c)
+    dv = E.dirichlet_values()
+    fs.dirichlet_commit(dv)
+    f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    op.function(dev(np.zeros(fs.nowned)), f)
+    F = f.cpu().numpy()
+    assert np.linalg.norm(F) == pytest.approx(HIST[name]["snes"][0], rel=6e-6)
+    assert rel(F, E.function(E.closure(np.zeros(fs.nowned), dv))[:fs.nowned]) < TOL64
+
+
+# ------------------------------------------------- nonlinear residual and Jv
+def sine_state(X, seed):
+    """Seeded sum of 8 random-phase sine modes of amplitude 0.1."""
+    r = splitmix64_uniform(seed, 32)
+    u = np.zeros(len(X))
+    for m in range(8):
+        k = 1 + np.floor(r[4 * m:4 * m + 3] * 4)
+        u += 0.1 * np.sin(2 * np.pi * (X @ k) + 2 * np.pi * r[4 * m + 3])
+    return u
+
+
+@pytest.mark.parametrize("M,p,lam", [((3, 3, 3), 6, 0.0), ((2, 3, 2), 3, 0.5), ((2, 2, 2), 8, 0.0)])
+def test_nonlinear_function_and_jacobian_action(C, M, p, lam):
+    print(f"nonlinear M={M} p={p} lam={lam}", flush=True)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=11)
+    fs, S, E, op = setup(C, M, p, V, p=1.5, eps=np.sqrt(0.005), lam=lam)
+    Xc = S.nodal_coordinates()
+    uc = sine_state(Xc, 3)
+    fs.dirichlet_commit(uc[fs.nowned:])
+    f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    op.function(dev(uc[:fs.nowned]), f)
+    assert rel(f.cpu().numpy(), E.function(uc)[:fs.nowned]) < TOL64
+    v = 2 * splitmix64_uniform(9, fs.nowned) - 1
+    y = torch.zeros_like(f)
+    op.mat_mult(dev(v), y)
+    ref = E.jacobian_apply(uc, np.concatenate([v, np.zeros(fs.nclosure - fs.nowned)]))[:fs.nowned]
+    assert rel(y.cpu().numpy(), ref) < TOL64
+    # FP32 inner apply: error against the FP64 oracle is stated, not zero
+    y32 = torch.zeros(fs.nowned, dtype=torch.float32, device="cuda")
+    op.mat_mult_f32(dev(v, torch.float32), y32)
+    e32 = rel(y32.cpu().numpy(), ref)
+    print(f"FP32 Jv relative error vs FP64 oracle: {e32:.3e}")
+    assert e32 < 1e-4
+
+
+def test_atomic_and_colored_scatter_agree(C):
+    M, p = (4, 4, 4), 5
+    V = block_vertices(M, perturb=0.1, seed=2)
+    fs = C.FS(M, p, V)
+    op = C.Ellip(fs, p=2.0)
+    x = dev(2 * splitmix64_uniform(4, fs.nowned) - 1)
+    y0 = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    y1 = torch.zeros_like(y0)
+    op.mat_mult(x, y0)
+    op.set_scatter(1)
+    op.mat_mult(x, y1)
+    assert rel(y1.cpu().numpy(), y0.cpu().numpy()) < 1e-14
+    # the colored scatter is deterministic: bitwise identical on repeat
+    op.set_scatter(0)
+    y2 = torch.zeros_like(y0)
+    op.mat_mult(x, y2)
+    assert torch.equal(y0, y2)
+
+
+@pytest.mark.parametrize("p", list(range(1, 11)))
+def test_poisson_apply_all_orders(C, p):
+    M = (2, 2, 2)
+    V = block_vertices(M, perturb=0.1, seed=p)
+    fs, S, E, op = setup(C, M, p, V, p=2.0)
+    x = 2 * splitmix64_uniform(p, fs.nowned) - 1
+    y = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    op.mat_mult(dev(x), y)
+    uc0 = np.zeros(fs.nclosure)
+    ref = E.jacobian_apply(uc0, np.concatenate([x, np.zeros(fs.nclosure - fs.nowned)]))[:fs.nowned]
+    assert rel(y.cpu().numpy(), ref) < TOL64
+
+
+def test_variable_coefficient(C):
+    M, p = (3, 3, 3), 4
+    V = block_vertices(M, perturb=0.1, seed=8)
+    kv = np.exp(splitmix64_uniform(12, V.shape[0]) - 0.5)
+    fs = C.FS(M, p, V)
+    S = O.BlockSpace(M, p, V)
+    everts = np.array([[((e0 + (t >> 2)) * (M[1] + 1) + e1 + ((t >> 1) & 1)) * (M[2] + 1) + e2 + (t & 1)
+                        for t in range(8)] for e0 in range(M[0]) for e1 in range(M[1]) for e2 in range(M[2])])
+    E = O.Ellip(S, p=2.0, kvert=kv[everts])
+    op = C.Ellip(fs, p=2.0, kvert=kv)
+    x = 2 * splitmix64_uniform(13, fs.nowned) - 1
+    y = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    op.mat_mult(dev(x), y)
+    ref = E.jacobian_apply(np.zeros(fs.nclosure), np.concatenate([x, np.zeros(fs.nclosure - fs.nowned)]))
+    assert rel(y.cpu().numpy(), ref[:fs.nowned]) < TOL64
+
+
+def test_negative_jacobian_is_an_error(C):
+    M = (2, 2, 2)
+    V = block_vertices(M)
+    V[:, 0] *= -1  # mirrored block: every element is left-handed, det J < 0 everywhere
+    fs = C.FS(M, 2, V)
+    with pytest.raises(C.DohpError) as ei:
+        C.Ellip(fs, p=2.0)
+    assert ei.value.code == C.ERR_NEGATIVE_JACOBIAN
+
+
+def test_fs_numbering_gpu_matches_oracle(C):
+    V = block_vertices((3, 2, 4), perturb=0.1, seed=3)
+    fs = C.FS((3, 2, 4), 3, V)
+    S = O.BlockSpace((3, 2, 4), 3, V)
+    np.testing.assert_array_equal(fs.element_nodes(), S.elem_nodes)

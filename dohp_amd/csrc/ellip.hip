@@ -22,12 +22,22 @@ namespace {
 
 enum { MODE_LIN = 0, MODE_FUN = 1, MODE_JAC = 2 };
 
+// Row stride of the LDS tables and element tiles: even, so rows start 16-byte aligned.
+constexpr int padded(int N) { return (N + 1) & ~1; }
+
 template <int N, typename T>
 struct Tab {
-  T B[N * N];    // B[q][j]: GLL nodal basis at Gauss point q
-  T Dh[N * N];   // Dh[q][r]: derivative at Gauss point q of the Gauss-point Lagrange basis r
   double w[N];   // Gauss weights
   double phi[N]; // (1 + z_q)/2
+};
+
+// Basis tables staged in LDS once per workgroup, rows padded to NP entries:
+//   B[q][j] (GLL nodal basis at Gauss point q), Dh[q][r] (collocated derivative at the Gauss
+//   points), and their transposes, so that every row or column a thread needs is contiguous.
+template <int N, typename T>
+struct LdsTab {
+  static constexpr int NP = padded(N);
+  T B[N * NP], Dh[N * NP], BT[N * NP], DhT[N * NP];
 };
 
 struct Phys {
@@ -83,96 +93,158 @@ __device__ inline long xcd_block(long b, long nb)
 }
 
 // Forward: column values col[] at the nodes -> uq (Gauss points) and reference gradient.
+// s1/s2 are this element's LDS tiles [i][j][k] with column stride NP.
 template <int N, typename T>
-__device__ inline void interp_grad(const Tab<N, T> &tb, const T (&col)[N], T (&uq)[N], T (&gx)[N], T (&gy)[N],
+__device__ inline void interp_grad(const LdsTab<N, T> &tb, const T (&col)[N], T (&uq)[N], T (&gx)[N], T (&gy)[N],
                                    T (&gz)[N], T *s1, T *s2, int i, int j)
 {
-  T t[N];
+  constexpr int NP = padded(N);
+  T t[N], row[N];
 #pragma unroll
   for (int q = 0; q < N; q++) {
     T s = 0;
 #pragma unroll
-    for (int k = 0; k < N; k++) s += tb.B[q * N + k] * col[k];
+    for (int k = 0; k < N; k++) s += tb.B[q * NP + k] * col[k];
     t[q] = s;
   }
 #pragma unroll
-  for (int k = 0; k < N; k++) s1[(i * N + j) * N + k] = t[k];
+  for (int k = 0; k < N; k++) s1[(i * N + j) * NP + k] = t[k];
+#pragma unroll
+  for (int m = 0; m < N; m++) row[m] = tb.B[j * NP + m];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    T s = 0;
+  for (int k = 0; k < N; k++) t[k] = 0;
 #pragma unroll
-    for (int jj = 0; jj < N; jj++) s += tb.B[j * N + jj] * s1[(i * N + jj) * N + k];
-    t[k] = s;
-  }
+  for (int m = 0; m < N; m++)
 #pragma unroll
-  for (int k = 0; k < N; k++) s2[(i * N + j) * N + k] = t[k];
+    for (int k = 0; k < N; k++) t[k] += row[m] * s1[(i * N + m) * NP + k];
+#pragma unroll
+  for (int k = 0; k < N; k++) s2[(i * N + j) * NP + k] = t[k];
+#pragma unroll
+  for (int m = 0; m < N; m++) row[m] = tb.B[i * NP + m];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    T s = 0;
+  for (int k = 0; k < N; k++) uq[k] = 0;
 #pragma unroll
-    for (int ii = 0; ii < N; ii++) s += tb.B[i * N + ii] * s2[(ii * N + j) * N + k];
-    uq[k] = s;
-  }
+  for (int m = 0; m < N; m++)
+#pragma unroll
+    for (int k = 0; k < N; k++) uq[k] += row[m] * s2[(m * N + j) * NP + k];
 #pragma unroll
   for (int q = 0; q < N; q++) {
     T s = 0;
 #pragma unroll
-    for (int k = 0; k < N; k++) s += tb.Dh[q * N + k] * uq[k];
+    for (int k = 0; k < N; k++) s += tb.Dh[q * NP + k] * uq[k];
     gz[q] = s;
   }
 #pragma unroll
-  for (int k = 0; k < N; k++) s1[(i * N + j) * N + k] = uq[k];
+  for (int k = 0; k < N; k++) s1[(i * N + j) * NP + k] = uq[k];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    T sy = 0, sx = 0;
+  for (int k = 0; k < N; k++) gy[k] = gx[k] = 0;
 #pragma unroll
-    for (int m = 0; m < N; m++) {
-      sy += tb.Dh[j * N + m] * s1[(i * N + m) * N + k];
-      sx += tb.Dh[i * N + m] * s1[(m * N + j) * N + k];
+  for (int m = 0; m < N; m++) {
+    const T dj = tb.Dh[j * NP + m], di = tb.Dh[i * NP + m];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      gy[k] += dj * s1[(i * N + m) * NP + k];
+      gx[k] += di * s1[(m * N + j) * NP + k];
     }
-    gy[k] = sy;
-    gx[k] = sx;
   }
 }
 
-// Geometry at the Gauss point (z_i, z_j, z_k) of a column: J = [c0 c1 c2] with c0, c1 linear
-// along the column and c2 constant; r_m are the rows of adj(J), so J^{-1}[m][:] = r_m / det.
-__device__ inline void column_geometry(const double (&Rc)[2][3], const double (&Sc)[2][3], const double (&c2)[3],
-                                       double g1, double (&r0)[3], double (&r1)[3], double (&r2)[3], double &det)
+template <int N, typename T>
+__device__ inline T tb_l(const T *tab, int r, int c) { return tab[r * padded(N) + c]; }
+
+// Polynomial form of the column geometry: with c0, c1 linear in g = (1+z_k)/2 and c2 constant,
+// r0 = c1 x c2 and r1 = c2 x c0 are linear in g, r2 = c0 x c1 and det = c0 . r0 quadratic.  The
+// coefficients are formed in FP64 from the vertex coordinates, once per column; evaluating them
+// costs 14 multiply-adds per Gauss point.
+template <typename G>
+struct ColGeo {
+  G p0[3], q0[3], p1[3], q1[3], p2[3], q2[3], s2[3], e0, e1, e2;
+  __device__ inline void eval(G g, G (&r0)[3], G (&r1)[3], G (&r2)[3], G &det) const
+  {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      r0[d] = p0[d] + g * q0[d];
+      r1[d] = p1[d] + g * q1[d];
+      r2[d] = p2[d] + g * (q2[d] + g * s2[d]);
+    }
+    det = e0 + g * (e1 + g * e2);
+  }
+};
+
+__device__ inline void cross3(const double (&a)[3], const double (&b)[3], double (&c)[3])
 {
-  const double g0 = 1.0 - g1;
-  double c0[3], c1[3];
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <typename G>
+__device__ inline ColGeo<G> make_colgeo(const double (&Rc)[2][3], const double (&Sc)[2][3], const double (&c2)[3])
+{
+  double a0[3], b0[3], a1[3], b1[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    c0[d] = Rc[0][d] * g0 + Rc[1][d] * g1;
-    c1[d] = Sc[0][d] * g0 + Sc[1][d] * g1;
+    a0[d] = Rc[0][d];
+    b0[d] = Rc[1][d] - Rc[0][d];
+    a1[d] = Sc[0][d];
+    b1[d] = Sc[1][d] - Sc[0][d];
   }
-  r0[0] = c1[1] * c2[2] - c1[2] * c2[1]; r0[1] = c1[2] * c2[0] - c1[0] * c2[2]; r0[2] = c1[0] * c2[1] - c1[1] * c2[0];
-  r1[0] = c2[1] * c0[2] - c2[2] * c0[1]; r1[1] = c2[2] * c0[0] - c2[0] * c0[2]; r1[2] = c2[0] * c0[1] - c2[1] * c0[0];
-  r2[0] = c0[1] * c1[2] - c0[2] * c1[1]; r2[1] = c0[2] * c1[0] - c0[0] * c1[2]; r2[2] = c0[0] * c1[1] - c0[1] * c1[0];
-  det = c0[0] * r0[0] + c0[1] * r0[1] + c0[2] * r0[2];
+  double p0[3], q0[3], p1[3], q1[3], p2[3], t0[3], t1[3], s2[3];
+  cross3(a1, c2, p0);
+  cross3(b1, c2, q0);
+  cross3(c2, a0, p1);
+  cross3(c2, b0, q1);
+  cross3(a0, a1, p2);
+  cross3(a0, b1, t0);
+  cross3(b0, a1, t1);
+  cross3(b0, b1, s2);
+  ColGeo<G> cg;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    cg.p0[d] = (G)p0[d]; cg.q0[d] = (G)q0[d];
+    cg.p1[d] = (G)p1[d]; cg.q1[d] = (G)q1[d];
+    cg.p2[d] = (G)p2[d]; cg.q2[d] = (G)(t0[d] + t1[d]); cg.s2[d] = (G)s2[d];
+  }
+  cg.e0 = (G)(a0[0] * p0[0] + a0[1] * p0[1] + a0[2] * p0[2]);
+  cg.e1 = (G)(a0[0] * q0[0] + a0[1] * q0[1] + a0[2] * q0[2] + b0[0] * p0[0] + b0[1] * p0[1] + b0[2] * p0[2]);
+  cg.e2 = (G)(b0[0] * q0[0] + b0[1] * q0[1] + b0[2] * q0[2]);
+  return cg;
 }
+
+// Reciprocal of a positive determinant: hardware estimate refined by Newton steps to full
+// precision (FP64) or used as is (FP32, 1 ulp).
+__device__ inline double recip(double d)
+{
+  double x = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, x, 1.0);
+  x = fma(x, e, x);
+  e = fma(-d, x, 1.0);
+  return fma(x, e, x);
+}
+__device__ inline float recip(float d) { return __builtin_amdgcn_rcpf(d); }
 
 template <int N, typename T, int MODE>
 __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elist, long ne,
                                                     const int *__restrict__ enodes, const int *__restrict__ everts,
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
                                                     const T *__restrict__ ustate, const T *__restrict__ xdir,
-                                                    T *__restrict__ y, long nowned, int atomic, Tab<N, T> tb, Phys ph)
+                                                    T *__restrict__ y, long nowned, int atomic,
+                                                    const T *__restrict__ gtab, Tab<N, T> tb, Phys ph)
 {
-  constexpr int NN = N * N, N3 = NN * N, EPB = elems_per_block(N);
-  constexpr int N3X = MODE == MODE_LIN ? 1 : EPB * N3;
-  __shared__ T S1[EPB * N3], S2[EPB * N3], S3[N3X];
+  constexpr int NN = N * N, N3 = NN * N, EPB = elems_per_block(N), NP = padded(N), CS = NN * NP;
+  constexpr int N3X = MODE == MODE_LIN ? 1 : EPB * CS;
+  __shared__ LdsTab<N, T> TB;
+  __shared__ T S1[EPB * CS], S2[EPB * CS], S3[N3X];
   __shared__ int IDX[EPB * N3];
   __shared__ double VX[EPB * 8 * 4];
 
   const long lb = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int le = tid / NN, ij = tid - le * NN, i = ij / N, j = ij - i * N;
-
+  for (int t = tid; t < 4 * N * NP; t += blockDim.x) (&TB.B[0])[t] = gtab[t];
   for (int t = tid; t < EPB * N3; t += blockDim.x) {
     const int l = t / N3;
     const long es = lb * EPB + l;
@@ -211,26 +283,19 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
       if (MODE == MODE_JAC) { a = ustate[c]; b = c < nowned ? xdir[c] : T(0); }
       if (MODE == MODE_LIN) a = c < nowned ? xdir[c] : T(0);
     }
-    S1[t] = a;
-    if (MODE == MODE_JAC) S3[t] = b;
+    const int l = t / N3, r = t - l * N3, col = r / N, k = r - col * N;
+    S1[l * CS + col * NP + k] = a;
+    if (MODE == MODE_JAC) S3[l * CS + col * NP + k] = b;
   }
   __syncthreads();
-  T *s1 = &S1[le * N3], *s2 = &S2[le * N3];
+  T *s1 = &S1[le * CS], *s2 = &S2[le * CS];
   T ucol[N], xcol[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    ucol[k] = s1[(i * N + j) * N + k];
-    if (MODE == MODE_JAC) xcol[k] = S3[le * N3 + (i * N + j) * N + k];
+    ucol[k] = s1[(i * N + j) * NP + k];
+    if (MODE == MODE_JAC) xcol[k] = S3[le * CS + (i * N + j) * NP + k];
   }
   __syncthreads();
-
-  T uq[N], gx[N], gy[N], gz[N];
-  interp_grad<N, T>(tb, ucol, uq, gx, gy, gz, s1, s2, i, j);
-  T xq[N], hx[N], hy[N], hz[N];
-  if (MODE == MODE_JAC) {
-    __syncthreads();
-    interp_grad<N, T>(tb, xcol, xq, hx, hy, hz, s1, s2, i, j);
-  }
 
   // geometry of the trilinear map along this thread's column (xi0 = z_i, xi1 = z_j)
   const double *X = &VX[le * 32];
@@ -253,50 +318,55 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
     }
   }
   const double c2[3] = {0.5 * (Pc[1][0] - Pc[0][0]), 0.5 * (Pc[1][1] - Pc[0][1]), 0.5 * (Pc[1][2] - Pc[0][2])};
-  const double wij = tb.w[i] * tb.w[j];
+  const ColGeo<T> cg = make_colgeo<T>(Rc, Sc, c2);
+  const T wij = (T)(tb.w[i] * tb.w[j]);
+
+  T uq[N], gx[N], gy[N], gz[N];
+  interp_grad<N, T>(TB, ucol, uq, gx, gy, gz, s1, s2, i, j);
 
   T vq[N], d0[N], d1[N], d2[N];
   if (MODE == MODE_LIN) {
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      double r0[3], r1[3], r2[3], det;
-      column_geometry(Rc, Sc, c2, tb.phi[k], r0, r1, r2, det);
-      const double idet = 1.0 / det;
-      const double g1 = tb.phi[k], g0 = 1.0 - g1;
-      const double kq = ph.has_k ? Kc[0] * g0 + Kc[1] * g1 : 1.0;
-      // Dv = jw k grad(x) = w k (sum_m ghat_m r_m);  df_m = jinv[m][:] . Dv
-      const double sc = wij * tb.w[k] * kq;
-      const double a0 = gx[k], a1 = gy[k], a2 = gz[k];
-      double Dv[3];
+      T r0[3], r1[3], r2[3], det;
+      cg.eval((T)tb.phi[k], r0, r1, r2, det);
+      const T kq = ph.has_k ? (T)(Kc[0] + (Kc[1] - Kc[0]) * tb.phi[k]) : T(1);
+      // Dv = jw k grad(x) = w k (sum_m ghat_m r_m) / det;  df_m = jinv[m][:] . Dv = r_m . Dv / det
+      const T sc = wij * (T)tb.w[k] * kq * recip(det);
+      const T a0 = gx[k], a1 = gy[k], a2 = gz[k];
+      T s3[3];
 #pragma unroll
-      for (int d = 0; d < 3; d++) Dv[d] = sc * (a0 * r0[d] + a1 * r1[d] + a2 * r2[d]);
+      for (int d = 0; d < 3; d++) s3[d] = a0 * r0[d] + a1 * r1[d] + a2 * r2[d];
       vq[k] = T(0);
-      d0[k] = (T)((r0[0] * Dv[0] + r0[1] * Dv[1] + r0[2] * Dv[2]) * idet);
-      d1[k] = (T)((r1[0] * Dv[0] + r1[1] * Dv[1] + r1[2] * Dv[2]) * idet);
-      d2[k] = (T)((r2[0] * Dv[0] + r2[1] * Dv[1] + r2[2] * Dv[2]) * idet);
+      d0[k] = sc * (r0[0] * s3[0] + r0[1] * s3[1] + r0[2] * s3[2]);
+      d1[k] = sc * (r1[0] * s3[0] + r1[1] * s3[1] + r1[2] * s3[2]);
+      d2[k] = sc * (r2[0] * s3[0] + r2[1] * s3[1] + r2[2] * s3[2]);
     }
   } else {
-    // Pointwise physics in three phases so that one copy of the transcendental code is
-    // emitted: (a) |grad u|^2 per point into this thread's own LDS slots, (b) eta, the
-    // Jacobian factor, forcing and Bratu term in a rolled loop, (c) the mapped fluxes.
-    // Own-slot LDS traffic needs no barrier (each thread reads only what it wrote).
+    // Pointwise physics of the state in three phases so that one copy of the transcendental
+    // code is emitted: (a) |grad u|^2 and jw per point into this thread's own LDS slots,
+    // (b) eta, the rank-one Jacobian factor, forcing and Bratu term in a rolled loop, (c) the
+    // per-point store.  Own-slot LDS traffic needs no barrier.
     __syncthreads();  // S1/S2/S3 are free once every column is in registers
-    T *E = &S1[le * N3 + (i * N + j) * N], *F1 = &S2[le * N3 + (i * N + j) * N], *U = &S3[le * N3 + (i * N + j) * N];
+    T *E = &S1[le * CS + (i * N + j) * NP], *F1 = &S2[le * CS + (i * N + j) * NP], *U = &S3[le * CS + (i * N + j) * NP];
+    T gu0[N], gu1[N], gu2[N];
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      double r0[3], r1[3], r2[3], det;
-      column_geometry(Rc, Sc, c2, tb.phi[k], r0, r1, r2, det);
-      const double idet = 1.0 / det;
-      const double a0 = gx[k], a1 = gy[k], a2 = gz[k];
-      double gu[3];
-#pragma unroll
-      for (int d = 0; d < 3; d++) gu[d] = (a0 * r0[d] + a1 * r1[d] + a2 * r2[d]) * idet;
-      E[k] = (T)(ph.eps2 + 0.5 * (gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2]));
+      T r0[3], r1[3], r2[3], det;
+      cg.eval((T)tb.phi[k], r0, r1, r2, det);
+      const T idet = recip(det);
+      const T a0 = gx[k], a1 = gy[k], a2 = gz[k];
+      gu0[k] = (a0 * r0[0] + a1 * r1[0] + a2 * r2[0]) * idet;
+      gu1[k] = (a0 * r0[1] + a1 * r1[1] + a2 * r2[1]) * idet;
+      gu2[k] = (a0 * r0[2] + a1 * r1[2] + a2 * r2[2]) * idet;
+      E[k] = (T)ph.eps2 + T(0.5) * (gu0[k] * gu0[k] + gu1[k] * gu1[k] + gu2[k] * gu2[k]);
+      F1[k] = wij * (T)tb.w[k] * det;  // jw
       U[k] = uq[k];
     }
 #pragma unroll 1
     for (int k = 0; k < N; k++) {
-      const T espg = E[k];
+      const T espg = E[k], jw = F1[k];
+      const T kq = ph.has_k ? (T)(Kc[0] + (Kc[1] - Kc[0]) * tb.phi[k]) : T(1);
       const T eta = pow(espg, T((ph.p - 2) / 2));
       const T lexp = ph.lambda != 0 ? T(ph.lambda) * exp(U[k]) : T(0);
       if (MODE == MODE_FUN) {
@@ -305,117 +375,132 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
           const double g1 = tb.phi[k], g0 = 1.0 - g1;
           f = forcing(ph, Pc[0][0] * g0 + Pc[1][0] * g1, Pc[0][1] * g0 + Pc[1][1] * g1, Pc[0][2] * g0 + Pc[1][2] * g1);
         }
-        F1[k] = (T)f;
-        U[k] = lexp;
+        E[k] = jw * eta * kq;               // Dv = jw eta k grad u
+        U[k] = -jw * (T)f + -jw * lexp;     // v = -jw f - jw lambda e^u
       } else {
-        F1[k] = sqrt(-T((ph.p - 2) / 2) * eta / espg);
-        U[k] = lexp;
+        // Dv = jw k (eta grad x - (s . grad x) s), s = sqrt(-(p-2)/2 eta / espg) grad u; folded:
+        // Dv = E grad x - (sv . grad x) sv with sv = sqrt(jw k) s
+        E[k] = jw * eta * kq;
+        F1[k] = sqrt(-T((ph.p - 2) / 2) * eta / espg * jw * kq);
+        U[k] = -jw * lexp;
       }
-      E[k] = eta;
     }
+    if (MODE == MODE_FUN) {
 #pragma unroll
-    for (int k = 0; k < N; k++) {
-      double r0[3], r1[3], r2[3], det;
-      column_geometry(Rc, Sc, c2, tb.phi[k], r0, r1, r2, det);
-      const double idet = 1.0 / det;
-      const double g1 = tb.phi[k], g0 = 1.0 - g1;
-      const double kq = ph.has_k ? Kc[0] * g0 + Kc[1] * g1 : 1.0;
-      const double jw = wij * tb.w[k] * det;
-      const double a0 = gx[k], a1 = gy[k], a2 = gz[k];
-      double gu[3], Dv[3], v;
-#pragma unroll
-      for (int d = 0; d < 3; d++) gu[d] = (a0 * r0[d] + a1 * r1[d] + a2 * r2[d]) * idet;
-      if (MODE == MODE_FUN) {
-        v = -jw * (double)F1[k];
-        v += -jw * (double)U[k];
-        const double sc = jw * (double)E[k] * kq;
-#pragma unroll
-        for (int d = 0; d < 3; d++) Dv[d] = sc * gu[d];
-      } else {
-        const double smdc = (double)F1[k] * sqrt(kq);
-        const double b0 = hx[k], b1 = hy[k], b2 = hz[k];
-        double gxd[3], smd[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          gxd[d] = (b0 * r0[d] + b1 * r1[d] + b2 * r2[d]) * idet;
-          smd[d] = smdc * gu[d];
-        }
-        const double dotw = (smd[0] * gxd[0] + smd[1] * gxd[1] + smd[2] * gxd[2]) * jw;
-        const double etaw = (double)E[k] * kq * jw;
-        v = -jw * (double)U[k] * (double)xq[k];
-#pragma unroll
-        for (int d = 0; d < 3; d++) Dv[d] = etaw * gxd[d] - dotw * smd[d];
+      for (int k = 0; k < N; k++) {
+        T r0[3], r1[3], r2[3], det;
+        cg.eval((T)tb.phi[k], r0, r1, r2, det);
+        const T sc = E[k];
+        const T Dv[3] = {sc * gu0[k], sc * gu1[k], sc * gu2[k]};
+        const T idet = recip(det);
+        vq[k] = U[k];
+        d0[k] = (r0[0] * Dv[0] + r0[1] * Dv[1] + r0[2] * Dv[2]) * idet;
+        d1[k] = (r1[0] * Dv[0] + r1[1] * Dv[1] + r1[2] * Dv[2]) * idet;
+        d2[k] = (r2[0] * Dv[0] + r2[1] * Dv[1] + r2[2] * Dv[2]) * idet;
       }
-      // transpose mapping: df_m = sum_e jinv[m][e] Dv[e] (efstopo.c:589-623)
-      vq[k] = (T)v;
-      d0[k] = (T)((r0[0] * Dv[0] + r0[1] * Dv[1] + r0[2] * Dv[2]) * idet);
-      d1[k] = (T)((r1[0] * Dv[0] + r1[1] * Dv[1] + r1[2] * Dv[2]) * idet);
-      d2[k] = (T)((r2[0] * Dv[0] + r2[1] * Dv[1] + r2[2] * Dv[2]) * idet);
+    } else {
+      // store: etaw, sv (3), lamjw per point; the state's fields are dead after this
+      T etaw[N], sv0[N], sv1[N], sv2[N], lam[N];
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        const T c1 = F1[k];
+        etaw[k] = E[k];
+        sv0[k] = c1 * gu0[k];
+        sv1[k] = c1 * gu1[k];
+        sv2[k] = c1 * gu2[k];
+        lam[k] = U[k];
+      }
+      __syncthreads();
+      T xq[N], hx[N], hy[N], hz[N];
+      interp_grad<N, T>(TB, xcol, xq, hx, hy, hz, s1, s2, i, j);
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        T r0[3], r1[3], r2[3], det;
+        cg.eval((T)tb.phi[k], r0, r1, r2, det);
+        const T idet = recip(det);
+        const T b0 = hx[k], b1 = hy[k], b2 = hz[k];
+        T gxd[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) gxd[d] = (b0 * r0[d] + b1 * r1[d] + b2 * r2[d]) * idet;
+        const T dot = sv0[k] * gxd[0] + sv1[k] * gxd[1] + sv2[k] * gxd[2];
+        const T Dv[3] = {etaw[k] * gxd[0] - dot * sv0[k], etaw[k] * gxd[1] - dot * sv1[k],
+                         etaw[k] * gxd[2] - dot * sv2[k]};
+        vq[k] = lam[k] * xq[k];
+        d0[k] = (r0[0] * Dv[0] + r0[1] * Dv[1] + r0[2] * Dv[2]) * idet;
+        d1[k] = (r1[0] * Dv[0] + r1[1] * Dv[1] + r1[2] * Dv[2]) * idet;
+        d2[k] = (r2[0] * Dv[0] + r2[1] * Dv[1] + r2[2] * Dv[2]) * idet;
+      }
     }
   }
 
   // transposed collocated derivative, combined at the Gauss points
-  T t[N];
+  T t[N], row[N], rw2[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
     T s = vq[k];
 #pragma unroll
-    for (int m = 0; m < N; m++) s += tb.Dh[m * N + k] * d2[m];
+    for (int m = 0; m < N; m++) s += tb_l<N, T>(TB.DhT, k, m) * d2[m];
     t[k] = s;
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    s1[(i * N + j) * N + k] = d1[k];
-    s2[(i * N + j) * N + k] = d0[k];
+    s1[(i * N + j) * NP + k] = d1[k];
+    s2[(i * N + j) * NP + k] = d0[k];
+  }
+#pragma unroll
+  for (int m = 0; m < N; m++) {
+    row[m] = TB.DhT[j * NP + m];
+    rw2[m] = TB.DhT[i * NP + m];
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    T s = t[k];
+  for (int m = 0; m < N; m++)
 #pragma unroll
-    for (int m = 0; m < N; m++) s += tb.Dh[m * N + j] * s1[(i * N + m) * N + k] + tb.Dh[m * N + i] * s2[(m * N + j) * N + k];
-    t[k] = s;
-  }
+    for (int k = 0; k < N; k++) t[k] += row[m] * s1[(i * N + m) * NP + k] + rw2[m] * s2[(m * N + j) * NP + k];
   // B^T along z, y, x
   T t2[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
     T s = 0;
 #pragma unroll
-    for (int q = 0; q < N; q++) s += tb.B[q * N + k] * t[q];
+    for (int q = 0; q < N; q++) s += tb_l<N, T>(TB.BT, k, q) * t[q];
     t2[k] = s;
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) s1[(i * N + j) * N + k] = t2[k];
+  for (int k = 0; k < N; k++) s1[(i * N + j) * NP + k] = t2[k];
+#pragma unroll
+  for (int m = 0; m < N; m++) row[m] = TB.BT[j * NP + m];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    T s = 0;
+  for (int k = 0; k < N; k++) t2[k] = 0;
 #pragma unroll
-    for (int q = 0; q < N; q++) s += tb.B[q * N + j] * s1[(i * N + q) * N + k];
-    t2[k] = s;
-  }
+  for (int m = 0; m < N; m++)
 #pragma unroll
-  for (int k = 0; k < N; k++) s2[(i * N + j) * N + k] = t2[k];
+    for (int k = 0; k < N; k++) t2[k] += row[m] * s1[(i * N + m) * NP + k];
+#pragma unroll
+  for (int k = 0; k < N; k++) s2[(i * N + j) * NP + k] = t2[k];
+#pragma unroll
+  for (int m = 0; m < N; m++) row[m] = TB.BT[i * NP + m];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    T s = 0;
+  for (int k = 0; k < N; k++) t[k] = 0;
 #pragma unroll
-    for (int q = 0; q < N; q++) s += tb.B[q * N + i] * s2[(q * N + j) * N + k];
-    t[k] = s;
-  }
+  for (int m = 0; m < N; m++)
 #pragma unroll
-  for (int k = 0; k < N; k++) s1[(i * N + j) * N + k] = t[k];
+    for (int k = 0; k < N; k++) t[k] += row[m] * s2[(m * N + j) * NP + k];
+#pragma unroll
+  for (int k = 0; k < N; k++) s1[(i * N + j) * NP + k] = t[k];
   __syncthreads();
   // scatter (coalesced over the element's node list); Dirichlet rows are discarded
   for (int tt = tid; tt < EPB * N3; tt += blockDim.x) {
     const int c = IDX[tt];
     if (c >= 0 && c < nowned) {
-      if (atomic) atomicAdd(&y[c], S1[tt]);
-      else y[c] += S1[tt];
+      const int l = tt / N3, r = tt - l * N3, col = r / N, k = r - col * N;
+      const T val = S1[l * CS + col * NP + k];
+      if (atomic) atomicAdd(&y[c], val);
+      else y[c] += val;
     }
   }
 }
@@ -477,10 +562,10 @@ __global__ void closure_kernel(const double *__restrict__ u, const double *__res
 struct _p_dohpEllip {
   dohpFS fs = nullptr;
   dohpEllipParam prm{};
-  int scatter = 0;  // 0 = colours, 1 = atomics
+  int scatter = 1;  // 0 = colours (deterministic), 1 = atomics
   bool has_state = false;
-  DevBuf<double> state, kvert;
-  DevBuf<float> state32;
+  DevBuf<double> state, kvert, tab64;
+  DevBuf<float> state32, tab32;
   // host tables
   std::vector<double> B, Dh, w, phi;
 };
@@ -491,16 +576,19 @@ template <int N, typename T>
 Tab<N, T> make_tab(const _p_dohpEllip *op)
 {
   Tab<N, T> tb;
-  for (int q = 0; q < N * N; q++) {
-    tb.B[q] = (T)op->B[q];
-    tb.Dh[q] = (T)op->Dh[q];
-  }
   for (int q = 0; q < N; q++) {
     tb.w[q] = op->w[q];
     tb.phi[q] = op->phi[q];
   }
   return tb;
 }
+
+template <typename T>
+const T *gtab(const _p_dohpEllip *op);
+template <>
+const double *gtab<double>(const _p_dohpEllip *op) { return op->tab64.ptr; }
+template <>
+const float *gtab<float>(const _p_dohpEllip *op) { return op->tab32.ptr; }
 
 Phys make_phys(const _p_dohpEllip *op)
 {
@@ -528,14 +616,14 @@ dErr launch_n(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_
     const long ne = fs->nelem;
     hipLaunchKernelGGL((ellip_kernel<N, T, MODE>), dim3((ne + EPB - 1) / EPB), block, 0, st, nullptr, ne,
                        fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr, fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y,
-                       fs->nowned, 1, tb, ph);
+                       fs->nowned, 1, gtab<T>(op), tb, ph);
   } else {
     for (int col = 0; col < 8; col++) {
       const long ne = fs->color_off[col + 1] - fs->color_off[col];
       if (!ne) continue;
       hipLaunchKernelGGL((ellip_kernel<N, T, MODE>), dim3((ne + EPB - 1) / EPB), block, 0, st,
                          fs->d_color_elems.ptr + fs->color_off[col], ne, fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr,
-                         fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 0, tb, ph);
+                         fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), tb, ph);
     }
   }
   DOHP_HIP(hipGetLastError());
@@ -610,6 +698,20 @@ extern "C" dErr dohpEllipCreate(dohpFS fs, const dohpEllipParam *prm, dohpEllip 
         set_error("Negative Jacobian at element " + std::to_string(e));
         err = dohpERR_NEGATIVE_JACOBIAN;
       }
+  }
+  {  // B, Dh, B^T, Dh^T with rows padded to an even length
+    const int NP = (N + 1) & ~1;
+    std::vector<double> t(4 * N * NP, 0.0);
+    for (int q = 0; q < N; q++)
+      for (int r = 0; r < N; r++) {
+        t[0 * N * NP + q * NP + r] = op->B[q * N + r];
+        t[1 * N * NP + q * NP + r] = op->Dh[q * N + r];
+        t[2 * N * NP + r * NP + q] = op->B[q * N + r];
+        t[3 * N * NP + r * NP + q] = op->Dh[q * N + r];
+      }
+    std::vector<float> tf(t.begin(), t.end());
+    if (!err) err = op->tab64.upload(t.data(), t.size());
+    if (!err) err = op->tab32.upload(tf.data(), tf.size());
   }
   if (!err) err = op->state.alloc(fs->nclosure);
   if (!err) err = op->state32.alloc(fs->nclosure);

@@ -175,6 +175,7 @@ def test_atomic_and_colored_scatter_agree(C):
     x = dev(2 * splitmix64_uniform(4, fs.nowned) - 1)
     y0 = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
     y1 = torch.zeros_like(y0)
+    op.set_scatter(0)
     op.mat_mult(x, y0)
     op.set_scatter(1)
     op.mat_mult(x, y1)

@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(_HERE, "libdohp_amd.so")
+LIBPATH = os.environ.get("DOHP_AMD_LIB", os.path.join(_HERE, "libdohp_amd.so"))
 
 APPLY_INTERP, APPLY_INTERP_TRANSPOSE, APPLY_GRAD, APPLY_GRAD_TRANSPOSE = 40001, 40002, 40003, 40004
 INSERT_VALUES, ADD_VALUES = 1, 2
@@ -65,6 +65,13 @@ _proto("dohpEllipFunction", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMult", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMultF32", _vp, _vp, _vp, _vp)
 _proto("dohpEllipSetScatter", _vp, ctypes.c_int)
+_proto("dohpEllipAssembleQ1", _vp, _vp, ctypes.POINTER(_vp), _vp)
+_proto("dohpMatGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long))
+_proto("dohpMatGetCSR", _vp, _ip, _ip, _dp)
+_proto("dohpMatMult", _vp, _vp, _vp, _vp)
+_proto("dohpMatMultF32", _vp, _vp, _vp, _vp)
+_proto("dohpMatGetDiagonal", _vp, _vp, _vp)
+_proto("dohpMatDestroy", ctypes.POINTER(_vp))
 _lib.dohpGetLastError.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 _lib.dohpGetLastError.restype = None
 
@@ -174,6 +181,46 @@ class FS:
             self.h = None
 
 
+# --------------------------------------------------------------- CSR matrix
+class Mat:
+    """Device CSR matrix (the assembled embedded-Q1 preconditioning matrix)."""
+
+    def __init__(self):
+        self.h = _vp()
+        self.nrows = self.nnz = 0
+
+    def _sizes(self):
+        a, b = _long(), _long()
+        check(_lib.dohpMatGetSizes(self.h, ctypes.byref(a), ctypes.byref(b)))
+        self.nrows, self.nnz = a.value, b.value
+
+    def csr(self):
+        rp = np.zeros(self.nrows + 1, dtype=np.int32)
+        col = np.zeros(self.nnz, dtype=np.int32)
+        val = np.zeros(self.nnz)
+        check(_lib.dohpMatGetCSR(self.h, rp.ctypes.data_as(_ip), col.ctypes.data_as(_ip), val.ctypes.data_as(_dp)))
+        return rp, col, val
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp, col, val = self.csr()
+        return sp.csr_matrix((val, col, rp), shape=(self.nrows, self.nrows))
+
+    def mult(self, x, y, stream=None):
+        check(_lib.dohpMatMult(self.h, _ptr(x), _ptr(y), _stream(stream)))
+
+    def mult_f32(self, x, y, stream=None):
+        check(_lib.dohpMatMultF32(self.h, _ptr(x), _ptr(y), _stream(stream)))
+
+    def diagonal(self, d, stream=None):
+        check(_lib.dohpMatGetDiagonal(self.h, _ptr(d), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.dohpMatDestroy(ctypes.byref(self.h))
+            self.h = None
+
+
 # ------------------------------------------------------------------- ellip
 class Ellip:
     """ellip's residual and MATSHELL Jacobian action on the GPU (src/fs/tests/ellip.c:407-472)."""
@@ -205,6 +252,13 @@ class Ellip:
 
     def mat_mult_f32(self, x, y, stream=None):
         check(_lib.dohpEllipShellMatMultF32(self.h, _ptr(x), _ptr(y), _stream(stream)))
+
+    def assemble_q1(self, u=None, mat=None, stream=None):
+        """Embedded-Q1 matrix at state u (EllipJacobian); reuses `mat` (same pattern) if given."""
+        m = mat if mat is not None else Mat()
+        check(_lib.dohpEllipAssembleQ1(self.h, _ptr(u), ctypes.byref(m.h), _stream(stream)))
+        m._sizes()
+        return m
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -79,6 +79,9 @@ struct DevBuf {
 
 }  // namespace dohp
 
+// Internal entry points shared between the operator and the Q1 assembly.
+extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hipStream_t st);
+
 // Function space on a structured hexahedral block.
 struct _p_dohpFS {
   int M[3] = {0, 0, 0};
@@ -92,7 +95,8 @@ struct _p_dohpFS {
   std::vector<int> elem_verts;       // [nelem][8], tensor order
   std::vector<int> color_off;        // 9 offsets into color_elems
   std::vector<int> color_elems;      // elements grouped by (ex%2, ey%2, ez%2)
-  dohp::DevBuf<int> d_elem_nodes, d_elem_verts, d_color_elems;
+  std::vector<int> trav_elems;       // all elements in Morton (Z-curve) order
+  dohp::DevBuf<int> d_elem_nodes, d_elem_verts, d_color_elems, d_trav_elems;
   dohp::DevBuf<double> d_vcoords, d_dcache;
   dohp::DevBuf<float> d_dcache32;
 };

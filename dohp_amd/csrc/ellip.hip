@@ -761,6 +761,27 @@ extern "C" dErr dohpEllipFunction(dohpEllip op, const double *u, double *f, void
   return launch<double, MODE_FUN>(op, op->state.ptr, nullptr, f, st);
 }
 
+extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hipStream_t st)
+{
+  const dohpFS fs = op->fs;
+  hipLaunchKernelGGL(closure_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
+                     fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
+  op->has_state = true;
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpEllipParam *prm, const double **kvert,
+                                              const double **state)
+{
+  DOHP_CHECK(op, dohpERR_ARG, "null operator");
+  *fs = op->fs;
+  *prm = op->prm;
+  *kvert = op->kvert.ptr;
+  *state = op->state.ptr;
+  return 0;
+}
+
 static bool is_linear(const _p_dohpEllip *op) { return op->prm.p == 2.0 && op->prm.lambda == 0.0; }
 
 extern "C" dErr dohpEllipShellMatMult(dohpEllip op, const double *x, double *y, void *stream)

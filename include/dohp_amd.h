@@ -110,8 +110,29 @@ dErr dohpEllipFunction(dohpEllip op, const double *u_dev, double *f_dev, void *s
  * MATOP_MULT body.  FP64 and FP32 variants; the FP32 one keeps geometry in FP64. */
 dErr dohpEllipShellMatMult(dohpEllip op, const double *x_dev, double *y_dev, void *stream);
 dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x_dev, float *y_dev, void *stream);
-/* Scatter strategy of the element kernels: 0 = eight-colour deterministic (default), 1 = atomics. */
+/* Scatter strategy of the element kernels: 0 = eight-colour deterministic (bitwise reproducible),
+ * 1 = hardware atomics (default; summation order varies at rounding level). */
 dErr dohpEllipSetScatter(dohpEllip op, int mode);
+
+/* -------------------------------------------------- embedded-Q1 preconditioner */
+typedef struct _p_dohpMat *dohpMat;
+
+/* Assemble the embedded-Q1 matrix on the GLL sub-cells (EllipJacobian, src/fs/tests/ellip.c:474-536,
+ * with dRulesetIteratorGetPatchAssembly, src/fs/interface/fsrulesetit.c:657-701, and
+ * dEFSGetExplicitSparse_Tensor_Hex, src/jacobi/impls/tensor/efstopo.c:660-717) at the state u
+ * (owned vector, Dirichlet values from the cache; NULL = the state of the last function
+ * evaluation) into a device CSR over owned nodes.  *mat == NULL creates the matrix (pattern:
+ * 27 lattice neighbours, dFSCreateMatrix, src/fs/interface/fs.c:770-813); otherwise it is
+ * refilled in place (SAME_NONZERO_PATTERN). */
+dErr dohpEllipAssembleQ1(dohpEllip op, const double *u_dev, dohpMat *mat, void *stream);
+dErr dohpMatGetSizes(dohpMat A, long *nrows, long *nnz);
+/* Copies the CSR arrays to host buffers (rowptr: nrows+1, col/val: nnz); any may be NULL. */
+dErr dohpMatGetCSR(dohpMat A, int *rowptr_host, int *col_host, double *val_host);
+/* y = A x in FP64, and in FP32 with the values rounded once at assembly (MatMult). */
+dErr dohpMatMult(dohpMat A, const double *x_dev, double *y_dev, void *stream);
+dErr dohpMatMultF32(dohpMat A, const float *x_dev, float *y_dev, void *stream);
+dErr dohpMatGetDiagonal(dohpMat A, double *d_dev, void *stream);
+dErr dohpMatDestroy(dohpMat *A);
 
 #ifdef __cplusplus
 }

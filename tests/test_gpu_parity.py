@@ -232,3 +232,49 @@ def test_fs_numbering_gpu_matches_oracle(C):
     fs = C.FS((3, 2, 4), 3, V)
     S = O.BlockSpace((3, 2, 4), 3, V)
     np.testing.assert_array_equal(fs.element_nodes(), S.elem_nodes)
+
+
+# ---------------------------------------------------------- embedded-Q1 matrix
+@pytest.mark.parametrize("M,deg,morph,ep,lam", [((4, 4, 4), 1, 0.0, 1.6, 0.0), ((4, 4, 4), 3, 0.0, 1.6, 0.0),
+                                                 ((4, 4, 4), 3, 1.0, 2.0, 0.0), ((3, 2, 3), 4, 0.5, 1.7, 1.0),
+                                                 ((2, 2, 2), 6, 0.3, 1.5, 0.0)])
+def test_q1_assembly_matches_oracle(C, M, deg, morph, ep, lam):
+    """Embedded-Q1 matrix incl. the +-1 derivative and the element-ordered inverse Jacobian."""
+    print(f"q1 M={M} p={deg} morph={morph}", flush=True)
+    V = block_vertices(M, morph=morph, twist=0.4 * morph, perturb=0.05, seed=4)
+    fs, S, E, op = setup(C, M, deg, V, p=ep, eps=1.0, lam=lam, gexact=0)
+    dv = E.dirichlet_values()
+    fs.dirichlet_commit(dv)
+    x = 0.3 * (2 * splitmix64_uniform(21, fs.nowned) - 1)
+    A = op.assemble_q1(dev(x)).to_scipy()
+    R = E.q1_matrix(E.closure(x, dv))
+    assert A.shape == R.shape
+    d = (A - R).toarray()
+    assert np.abs(d).max() <= 1e-12 * np.abs(R.toarray()).max()
+    # SpMV and diagonal on the device
+    mat = op.assemble_q1(dev(x))
+    z = 2 * splitmix64_uniform(22, fs.nowned) - 1
+    yd = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    mat.mult(dev(z), yd)
+    assert rel(yd.cpu().numpy(), R @ z) < 1e-13
+    dd = torch.zeros_like(yd)
+    mat.diagonal(dd)
+    np.testing.assert_allclose(dd.cpu().numpy(), R.diagonal(), rtol=1e-14)
+
+
+def test_q1_refout_preconditioned_norms_on_gpu(C):
+    """First preconditioned KSP residuals of the refouts with the GPU-assembled Q1 matrix."""
+    import scipy.sparse.linalg as spla
+    for name, (bdeg, ep, lam, abc, morph, twist) in {
+            "ellip-e0-b3-p16": (3, 1.6, 0.0, (1.6, 1.7, 1.8), 0.0, 0.0),
+            "ellip-morph": (3, 2.0, 0.0, (1, 1, 1), 1.0, 0.4)}.items():
+        V = block_vertices((4, 4, 4), morph=morph, twist=twist)
+        fs, S, E, op = setup(C, (4, 4, 4), bdeg, V, p=ep, eps=1.0, lam=lam, gexact=0, a=abc[0], b=abc[1], c=abc[2])
+        dv = E.dirichlet_values()
+        fs.dirichlet_commit(dv)
+        u0 = dev(np.zeros(fs.nowned))
+        f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+        op.function(u0, f)
+        A = op.assemble_q1(u0).to_scipy()
+        r0 = spla.splu(A.tocsc()).solve(f.cpu().numpy())
+        assert np.linalg.norm(r0) == pytest.approx(HIST[name]["ksp"][0][0], rel=6e-6)

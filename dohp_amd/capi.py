@@ -72,6 +72,19 @@ _proto("dohpMatMult", _vp, _vp, _vp, _vp)
 _proto("dohpMatMultF32", _vp, _vp, _vp, _vp)
 _proto("dohpMatGetDiagonal", _vp, _vp, _vp)
 _proto("dohpMatDestroy", ctypes.POINTER(_vp))
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("mixed", ctypes.c_int), ("rtol", ctypes.c_double), ("max_it", ctypes.c_int),
+                ("inner_rtol", ctypes.c_double), ("max_outer", ctypes.c_int), ("cheb_degree", ctypes.c_int),
+                ("cheb_ratio", ctypes.c_double), ("eig_its", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("outer_its", ctypes.c_int), ("inner_its", ctypes.c_int), ("bnorm", ctypes.c_double),
+                ("rnorm", ctypes.c_double), ("lmax", ctypes.c_double), ("setup_seconds", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+_proto("dohpEllipSolve", _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)
 _lib.dohpGetLastError.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 _lib.dohpGetLastError.restype = None
 
@@ -259,6 +272,15 @@ class Ellip:
         check(_lib.dohpEllipAssembleQ1(self.h, _ptr(u), ctypes.byref(m.h), _stream(stream)))
         m._sizes()
         return m
+
+    def solve(self, q1, b, x, mixed=False, rtol=1e-10, max_it=2000, inner_rtol=1e-4, max_outer=50, cheb_degree=4,
+              cheb_ratio=30.0, eig_its=20, stream=None):
+        """CG on the linear operator, preconditioned by Jacobi-Chebyshev on the embedded-Q1 matrix."""
+        o = SolveOpts(int(mixed), float(rtol), int(max_it), float(inner_rtol), int(max_outer), int(cheb_degree),
+                      float(cheb_ratio), int(eig_its))
+        st = SolveStats()
+        check(_lib.dohpEllipSolve(self.h, q1.h, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(st), _stream(stream)))
+        return {f: getattr(st, f) for f, _ in SolveStats._fields_}
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -1,0 +1,325 @@
+// Krylov drivers above the operator: the role PETSc's KSP plays for ellip
+// (src/fs/tests/ellip.c:701-747), on device vectors.
+//
+//   * PCG with the embedded-Q1 matrix as preconditioner, applied approximately by a
+//     Jacobi-preconditioned Chebyshev iteration of fixed degree (a fixed polynomial in
+//     D^{-1} A_q1 times D^{-1}, hence symmetric, so CG stays valid).
+//   * Plain path: everything in FP64.
+//   * Mixed path: FP64 outer defect correction x += A32^{-1}(b - A64 x) with the inner PCG in
+//     FP32 (FP32 operator and Q1 applies); residuals of the outer loop and every Krylov dot
+//     product accumulate in FP64.
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+using namespace dohp;
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+
+template <typename T>
+__global__ void dot_partial_kernel(long n, const T *__restrict__ a, const T *__restrict__ b, double *__restrict__ part)
+{
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    s += (double)a[i] * (double)b[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void dot_final_kernel(int nb, const double *__restrict__ part, double *__restrict__ out)
+{
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// y = a*x + b*y
+template <typename T>
+__global__ void axpby_kernel(long n, T a, const T *__restrict__ x, T b, T *__restrict__ y)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// z = s * d * r  (diagonal scaling)
+template <typename T>
+__global__ void dscale_kernel(long n, T s, const T *__restrict__ dinv, const T *__restrict__ r, T *__restrict__ z)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = s * dinv[i] * r[i];
+}
+
+__global__ void inv_kernel(long n, const double *__restrict__ d, double *__restrict__ di, float *__restrict__ di32)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    di[i] = 1.0 / d[i];
+    di32[i] = (float)di[i];
+  }
+}
+
+template <typename T>
+__global__ void fill_kernel(long n, T v, T *__restrict__ y)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+// y32 = s * x64 ; x64 += s * y32
+__global__ void demote_kernel(long n, double s, const double *__restrict__ x, float *__restrict__ y)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)(s * x[i]);
+}
+__global__ void promote_add_kernel(long n, double s, const float *__restrict__ y, double *__restrict__ x)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += s * (double)y[i];
+}
+
+inline dim3 grid(long n) { return dim3((n + 255) / 256); }
+
+struct Ctx {
+  hipStream_t st;
+  long n;
+  DevBuf<double> part, res;
+};
+
+template <typename T>
+dErr dot(Ctx &c, const T *a, const T *b, double *out)
+{
+  hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(kRedBlocks), dim3(256), 0, c.st, c.n, a, b, c.part.ptr);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, c.st, kRedBlocks, c.part.ptr, c.res.ptr);
+  DOHP_HIP(hipMemcpyAsync(out, c.res.ptr, sizeof(double), hipMemcpyDeviceToHost, c.st));
+  DOHP_HIP(hipStreamSynchronize(c.st));
+  return 0;
+}
+
+template <typename T>
+dErr axpby(Ctx &c, T a, const T *x, T b, T *y)
+{
+  hipLaunchKernelGGL(axpby_kernel<T>, grid(c.n), dim3(256), 0, c.st, c.n, a, x, b, y);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+// Operators of one precision.
+template <typename T>
+struct Ops;
+template <>
+struct Ops<double> {
+  static dErr A(dohpEllip op, const double *x, double *y, hipStream_t s) { return dohpEllipShellMatMult(op, x, y, s); }
+  static dErr Q(dohpMat m, const double *x, double *y, hipStream_t s) { return dohpMatMult(m, x, y, s); }
+};
+template <>
+struct Ops<float> {
+  static dErr A(dohpEllip op, const float *x, float *y, hipStream_t s) { return dohpEllipShellMatMultF32(op, x, y, s); }
+  static dErr Q(dohpMat m, const float *x, float *y, hipStream_t s) { return dohpMatMultF32(m, x, y, s); }
+};
+
+// Preconditioner: degree-k Chebyshev iteration on A_q1 z = r with Jacobi scaling, z0 = 0.
+template <typename T>
+struct Cheb {
+  dohpMat q1;
+  const T *dinv;
+  double lmin, lmax;
+  int degree;
+  DevBuf<T> res, d, w;
+  dErr apply(Ctx &c, const T *r, T *z)
+  {
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    hipLaunchKernelGGL(dscale_kernel<T>, grid(c.n), dim3(256), 0, c.st, c.n, (T)(1.0 / theta), dinv, r, d.ptr);
+    DOHP_HIP(hipMemcpyAsync(z, d.ptr, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
+    DOHP_HIP(hipMemcpyAsync(res.ptr, r, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
+    for (int k = 1; k < degree; k++) {
+      DOHP_CALL(Ops<T>::Q(q1, d.ptr, w.ptr, c.st));
+      DOHP_CALL(axpby<T>(c, T(-1), w.ptr, T(1), res.ptr));  // res = r - A z
+      const double rho_new = 1.0 / (2.0 * sigma - rho);
+      hipLaunchKernelGGL(dscale_kernel<T>, grid(c.n), dim3(256), 0, c.st, c.n, (T)(2.0 * rho_new / delta), dinv,
+                         res.ptr, w.ptr);
+      DOHP_CALL(axpby<T>(c, T(1), w.ptr, (T)(rho_new * rho), d.ptr));  // d = rho' rho d + 2 rho'/delta D^-1 res
+      DOHP_CALL(axpby<T>(c, T(1), d.ptr, T(1), z));
+      rho = rho_new;
+    }
+    return 0;
+  }
+};
+
+// Largest eigenvalue of D^{-1} A_q1 by power iteration (FP64).
+dErr estimate_lmax(Ctx &c, dohpMat q1, const double *dinv, int its, double *lmax)
+{
+  DevBuf<double> v, w;
+  DOHP_CALL(v.alloc(c.n));
+  DOHP_CALL(w.alloc(c.n));
+  std::vector<double> h(c.n);
+  uint64_t s = 0x9E3779B97F4A7C15ULL;
+  for (long i = 0; i < c.n; i++) {
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    h[i] = (double)(s >> 11) / 9007199254740992.0 - 0.5;
+  }
+  DOHP_HIP(hipMemcpy(v.ptr, h.data(), sizeof(double) * c.n, hipMemcpyHostToDevice));
+  double lam = 0, nv;
+  DOHP_CALL(dot(c, v.ptr, v.ptr, &nv));
+  DOHP_CALL(axpby<double>(c, 0.0, v.ptr, 1.0 / std::sqrt(nv), v.ptr));
+  for (int k = 0; k < its; k++) {
+    DOHP_CALL(dohpMatMult(q1, v.ptr, w.ptr, c.st));
+    hipLaunchKernelGGL(dscale_kernel<double>, grid(c.n), dim3(256), 0, c.st, c.n, 1.0, dinv, w.ptr, w.ptr);
+    double vw, ww;
+    DOHP_CALL(dot(c, v.ptr, w.ptr, &vw));
+    DOHP_CALL(dot(c, w.ptr, w.ptr, &ww));
+    lam = vw;
+    DOHP_CALL(axpby<double>(c, 1.0 / std::sqrt(ww), w.ptr, 0.0, v.ptr));
+  }
+  *lmax = lam;
+  return 0;
+}
+
+// PCG on A x = b (x holds the initial guess).  Stops at ||r|| <= rtol ||b|| or max_it.
+template <typename T>
+dErr pcg(Ctx &c, dohpEllip op, Cheb<T> &M, const T *b, T *x, double rtol, int max_it, int *its, double *rnorm,
+         double *bnorm)
+{
+  DevBuf<T> r, z, p, q;
+  DOHP_CALL(r.alloc(c.n));
+  DOHP_CALL(z.alloc(c.n));
+  DOHP_CALL(p.alloc(c.n));
+  DOHP_CALL(q.alloc(c.n));
+  DOHP_CALL(Ops<T>::A(op, x, q.ptr, c.st));
+  DOHP_HIP(hipMemcpyAsync(r.ptr, b, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
+  DOHP_CALL(axpby<T>(c, T(-1), q.ptr, T(1), r.ptr));
+  double bb, rr;
+  DOHP_CALL(dot(c, b, b, &bb));
+  DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
+  *bnorm = std::sqrt(bb);
+  const double tol = rtol * std::sqrt(bb);
+  int k = 0;
+  double rz = 0;
+  if (std::sqrt(rr) > tol) {
+    DOHP_CALL(M.apply(c, r.ptr, z.ptr));
+    DOHP_HIP(hipMemcpyAsync(p.ptr, z.ptr, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
+    DOHP_CALL(dot(c, r.ptr, z.ptr, &rz));
+  }
+  while (std::sqrt(rr) > tol && k < max_it) {
+    DOHP_CALL(Ops<T>::A(op, p.ptr, q.ptr, c.st));
+    double pq;
+    DOHP_CALL(dot(c, p.ptr, q.ptr, &pq));
+    const double alpha = rz / pq;
+    DOHP_CALL(axpby<T>(c, (T)alpha, p.ptr, T(1), x));
+    DOHP_CALL(axpby<T>(c, (T)-alpha, q.ptr, T(1), r.ptr));
+    DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
+    k++;
+    if (std::sqrt(rr) <= tol) break;
+    DOHP_CALL(M.apply(c, r.ptr, z.ptr));
+    double rz_new;
+    DOHP_CALL(dot(c, r.ptr, z.ptr, &rz_new));
+    DOHP_CALL(axpby<T>(c, T(1), z.ptr, (T)(rz_new / rz), p.ptr));
+    rz = rz_new;
+  }
+  *its = k;
+  *rnorm = std::sqrt(rr);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double *x, const dohpSolveOpts *opts,
+                               dohpSolveStats *stats, void *stream)
+{
+  DOHP_CHECK(op && q1 && b && x && opts && stats, dohpERR_ARG, "null argument");
+  const auto t0 = std::chrono::steady_clock::now();
+  Ctx c;
+  c.st = (hipStream_t)stream;
+  long nrows;
+  DOHP_CALL(dohpMatGetSizes(q1, &nrows, nullptr));
+  c.n = nrows;
+  DOHP_CALL(c.part.alloc(kRedBlocks));
+  DOHP_CALL(c.res.alloc(1));
+  *stats = dohpSolveStats{};
+  // Jacobi scaling and the Chebyshev interval [lmax/ratio, 1.05 lmax] of D^{-1} A_q1
+  DevBuf<double> diag, dinv;
+  DevBuf<float> dinv32;
+  DOHP_CALL(diag.alloc(c.n));
+  DOHP_CALL(dinv.alloc(c.n));
+  DOHP_CALL(dinv32.alloc(c.n));
+  DOHP_CALL(dohpMatGetDiagonal(q1, diag.ptr, c.st));
+  hipLaunchKernelGGL(inv_kernel, grid(c.n), dim3(256), 0, c.st, c.n, diag.ptr, dinv.ptr, dinv32.ptr);
+  double lmax;
+  DOHP_CALL(estimate_lmax(c, q1, dinv.ptr, opts->eig_its > 0 ? opts->eig_its : 20, &lmax));
+  lmax *= 1.05;
+  const double lmin = lmax / (opts->cheb_ratio > 1 ? opts->cheb_ratio : 30.0);
+  stats->lmax = lmax;
+  const int degree = opts->cheb_degree > 0 ? opts->cheb_degree : 4;
+  const auto t_setup = std::chrono::steady_clock::now();
+  stats->setup_seconds = std::chrono::duration<double>(t_setup - t0).count();
+  if (!opts->mixed) {
+    Cheb<double> M{q1, dinv.ptr, lmin, lmax, degree};
+    DOHP_CALL(M.res.alloc(c.n));
+    DOHP_CALL(M.d.alloc(c.n));
+    DOHP_CALL(M.w.alloc(c.n));
+    int its;
+    double rn, bn;
+    DOHP_CALL(pcg<double>(c, op, M, b, x, opts->rtol, opts->max_it, &its, &rn, &bn));
+    stats->inner_its = its;
+    stats->outer_its = 1;
+    stats->rnorm = rn;
+    stats->bnorm = bn;
+  } else {
+    Cheb<float> M{q1, dinv32.ptr, lmin, lmax, degree};
+    DOHP_CALL(M.res.alloc(c.n));
+    DOHP_CALL(M.d.alloc(c.n));
+    DOHP_CALL(M.w.alloc(c.n));
+    DevBuf<double> r, t;
+    DevBuf<float> r32, e32;
+    DOHP_CALL(r.alloc(c.n));
+    DOHP_CALL(t.alloc(c.n));
+    DOHP_CALL(r32.alloc(c.n));
+    DOHP_CALL(e32.alloc(c.n));
+    double bb;
+    DOHP_CALL(dot(c, b, b, &bb));
+    stats->bnorm = std::sqrt(bb);
+    int outer = 0, inner_total = 0;
+    for (;;) {
+      // FP64 defect r = b - A64 x
+      DOHP_CALL(dohpEllipShellMatMult(op, x, t.ptr, c.st));
+      DOHP_HIP(hipMemcpyAsync(r.ptr, b, sizeof(double) * c.n, hipMemcpyDeviceToDevice, c.st));
+      DOHP_CALL(axpby<double>(c, -1.0, t.ptr, 1.0, r.ptr));
+      double rr;
+      DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
+      stats->rnorm = std::sqrt(rr);
+      if (stats->rnorm <= opts->rtol * stats->bnorm || outer >= opts->max_outer) break;
+      // FP32 correction, normalized so the inner problem is O(1)
+      const double s = 1.0 / stats->rnorm;
+      hipLaunchKernelGGL(demote_kernel, grid(c.n), dim3(256), 0, c.st, c.n, s, r.ptr, r32.ptr);
+      hipLaunchKernelGGL(fill_kernel<float>, grid(c.n), dim3(256), 0, c.st, c.n, 0.0f, e32.ptr);
+      int its;
+      double rn, bn;
+      DOHP_CALL(pcg<float>(c, op, M, r32.ptr, e32.ptr, opts->inner_rtol, opts->max_it, &its, &rn, &bn));
+      inner_total += its;
+      hipLaunchKernelGGL(promote_add_kernel, grid(c.n), dim3(256), 0, c.st, c.n, stats->rnorm, e32.ptr, x);
+      outer++;
+    }
+    stats->outer_its = outer;
+    stats->inner_its = inner_total;
+  }
+  DOHP_HIP(hipStreamSynchronize(c.st));
+  stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}

@@ -134,6 +134,31 @@ dErr dohpMatMultF32(dohpMat A, const float *x_dev, float *y_dev, void *stream);
 dErr dohpMatGetDiagonal(dohpMat A, double *d_dev, void *stream);
 dErr dohpMatDestroy(dohpMat *A);
 
+/* ------------------------------------------------------------------ solvers */
+typedef struct {
+  int mixed;          /* 0: plain FP64 PCG; 1: FP32 inner PCG inside FP64 defect correction */
+  double rtol;        /* relative residual target ||b - A x|| <= rtol ||b|| (FP64 residual) */
+  int max_it;         /* PCG iteration limit (per inner solve when mixed) */
+  double inner_rtol;  /* inner FP32 PCG tolerance (mixed) */
+  int max_outer;      /* defect-correction sweeps limit (mixed) */
+  int cheb_degree;    /* degree of the Jacobi-Chebyshev iteration on the Q1 matrix */
+  double cheb_ratio;  /* Chebyshev interval [lmax/ratio, lmax] of D^{-1} A_q1 */
+  int eig_its;        /* power iterations for lmax */
+} dohpSolveOpts;
+
+typedef struct {
+  int outer_its, inner_its;
+  double bnorm, rnorm, lmax;
+  double setup_seconds, seconds;
+} dohpSolveStats;
+
+/* Solve J(u) x = b (the linear operator of the ellip context; x holds the initial guess) by CG
+ * preconditioned with the embedded-Q1 matrix under Jacobi-Chebyshev smoothing -- the KSP/PC
+ * role of ellip's driver (src/fs/tests/ellip.c:594-652,701-747).  Krylov dot products and the
+ * outer residual accumulate in FP64. */
+dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b_dev, double *x_dev, const dohpSolveOpts *opts,
+                    dohpSolveStats *stats, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,18 @@ p = 1.5, eps^2 = 0.005 has eta = 2^(1/4) k, so the operator is ellip's up to tha
 A step is one Jacobian-vector product y = J(u0) v (the MATSHELL MatMult a Krylov solver calls)
 in FP64 with v ~ U[-1,1] (seeded), inputs resident in HBM.  value = N_int / t_step in GDOF/s.
 The residual F(u0), the FP32 Jacobian action and the linear (Poisson) apply are timed too and
-reported under "variants".
+reported under "variants".  roofline.achieved divides the algorithmic bytes of one J v by the
+element kernel's own duration (y += J v, timed with HIP events on its stream); roofline.traffic
+is the HBM traffic of one launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, read from the
+profile summary committed under profiles/ when it covers this workload.
+
+--workload solve: variable-coefficient Poisson -div(k grad u) = f with log-normal k (unit
+variance, correlation length 0.1, seeded) on 48^3 elements, order 5; CG preconditioned by the
+embedded-Q1 matrix under Jacobi-Chebyshev, plain FP64 versus FP32-inner/FP64-outer defect
+correction, both to relative residual 1e-10; prints one JSON line with iterations, times and
+the final FP64 error against the manufactured solution.
+
+--copy-probe: one plain device copy of 1 GiB (the calibration kernel for the HBM counters).
 """
 import argparse
 import json
@@ -57,7 +68,7 @@ def time_fn(torch, fn, reps, warm=2):
 
 
 def cpu_baseline(nthreads):
-    """Oracle J(u) v on a bounded sample of the same workload: 12^3 elements, p=6."""
+    """Oracle J(u) v on a bounded sample of the same workload: 20^3 elements, p=6."""
     from dohp_amd.mesh import block_vertices, splitmix64_uniform
     from oracle import oracle as O
     M = (20, 20, 20)
@@ -76,27 +87,107 @@ def cpu_baseline(nthreads):
     return S.nowned / (t / reps) / 1e9, f"20^3 elements p=6 J(u)v, mean of {reps} applies"
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the element kernel, from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "ellip_jv_f64_64cube_p6_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                key, val = line.split(":", 1)[1].rsplit("=", 1)
+                if key.strip() == workload:
+                    return int(float(val))
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe():
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 3}))
+
+
+def solve_workload(args):
+    import torch
+    import dohp_amd.capi as C
+    from dohp_amd.fields import lognormal_field
+    from dohp_amd.mesh import block_vertices
+    Me, p = args.elems or 48, args.degree or 5
+    M = (Me, Me, Me)
+    t0 = time.perf_counter()
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1)
+    fs = C.FS(M, p, V)
+    k = lognormal_field(V, seed=11, corr_len=0.1)
+    op = C.Ellip(fs, p=2.0, kvert=k)
+    X = fs.nodal_coordinates()[:fs.nowned]
+    xs = torch.as_tensor(sine_state(X, 5) + X[:, 0] * (1 - X[:, 0]), dtype=torch.float64, device="cuda")
+    del X
+    b = torch.zeros_like(xs)
+    op.mat_mult(xs, b)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter()
+    q1 = op.assemble_q1(torch.zeros_like(xs))
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t_asm
+    print(f"# solve setup {time.perf_counter() - t0:.1f}s (Q1 assembly {t_asm * 1e3:.1f} ms): {fs.nelem} elements "
+          f"p={p}, N_int={fs.nowned}, nnz={q1.nnz}", file=sys.stderr, flush=True)
+    res = {}
+    for name, mixed in (("plain_f64", False), ("mixed_f32_f64", True)):
+        x = torch.zeros_like(xs)
+        op.solve(q1, b, x, mixed=mixed, rtol=1e-10)  # warm-up (kernels, allocations)
+        x.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = op.solve(q1, b, x, mixed=mixed, rtol=1e-10)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t
+        r = torch.zeros_like(x)
+        op.mat_mult(x, r)
+        res[name] = {"seconds": round(t, 4), "outer_its": st["outer_its"], "inner_its": st["inner_its"],
+                     "relres": float(torch.linalg.norm(b - r) / torch.linalg.norm(b)),
+                     "error_vs_exact": float(torch.linalg.norm(x - xs) / torch.linalg.norm(xs))}
+        print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
+    out = {"metric": "solve time to relative residual 1e-10", "value": res["mixed_f32_f64"]["seconds"], "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "f32 inner / f64 outer", "data": "synthetic",
+           "config": {"workload": f"log-normal k Poisson, {Me}^3 hex, p={p}, CG + embedded-Q1 Jacobi-Chebyshev",
+                      "n_int": fs.nowned, "q1_nnz": q1.nnz, "q1_assembly_ms": round(t_asm * 1e3, 2)},
+           "speedup_mixed_vs_plain": round(res["plain_f64"]["seconds"] / res["mixed_f32_f64"]["seconds"], 3),
+           "runs": res}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--elems", type=int, default=64)
-    ap.add_argument("--degree", type=int, default=6)
+    ap.add_argument("--elems", type=int, default=None)
+    ap.add_argument("--degree", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--scatter", type=int, default=0, help="0 = eight-colour deterministic, 1 = atomics")
+    ap.add_argument("--scatter", type=int, default=1, help="0 = eight-colour deterministic, 1 = atomics")
+    ap.add_argument("--workload", choices=["apply", "solve"], default="apply")
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU; this build has no multi-GPU path (see DESIGN.md, sharding).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: this operator has no multi-GPU path (its element-partitioned "
+              "sharding is a design, not code).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        return copy_probe()
+    if args.workload == "solve":
+        return solve_workload(args)
 
     import torch
     import dohp_amd.capi as C
     from dohp_amd.mesh import block_vertices, splitmix64_uniform
 
     dev = torch.device("cuda:0")
-    Me, p = args.elems, args.degree
+    Me, p = args.elems or 64, args.degree or 6
     M = (Me, Me, Me)
     n = p + 1
     t_setup = time.perf_counter()
@@ -135,6 +226,11 @@ def main():
     torch.cuda.synchronize()
     t_step = e0.elapsed_time(e1) / 1e3 / args.steps
     print(f"# Jv fp64 {t_step * 1e6:.1f} us", file=sys.stderr, flush=True)
+    # the element kernel alone (y += J v), same stream
+    t_kernel = None
+    if args.scatter == 1:
+        t_kernel = time_fn(torch, lambda: op.mat_mult_add(v, y, stream), args.steps, warm=1)
+        print(f"# Jv fp64 element kernel {t_kernel * 1e6:.1f} us", file=sys.stderr, flush=True)
 
     variants = {}
     reps = max(3, args.steps // 2)
@@ -154,7 +250,8 @@ def main():
     variants["jv_f32"]["rel_err_vs_f64"] = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
 
     B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True)
-    achieved = B / t_step / 1e9
+    achieved = B / (t_kernel or t_step) / 1e9
+    workload = f"nonlinear diffusion J(u0)v, {Me}^3 hex, p={p}, FP64, perturbed mesh"
     cpu = None
     if not args.no_cpu:
         ncores = os.cpu_count() or 1
@@ -177,11 +274,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"nonlinear diffusion J(u0)v, {Me}^3 hex, p={p}, FP64, perturbed mesh",
-                   "elements": fs.nelem, "degree": p, "n_int": fs.nowned, "n_tot": fs.nclosure,
-                   "scatter": "colored" if args.scatter == 0 else "atomic"},
+        "config": {"workload": workload, "elements": fs.nelem, "degree": p, "n_int": fs.nowned,
+                   "n_tot": fs.nclosure, "scatter": "colored" if args.scatter == 0 else "atomic"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload),
+                     "kernel_us": round((t_kernel or t_step) * 1e6, 1), "algorithmic_bytes": int(B)},
         "cpu_baseline": cpu,
         "variants": variants,
     }

@@ -64,6 +64,7 @@ _proto("dohpEllipSetCoefficient", _vp, _dp)
 _proto("dohpEllipFunction", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMult", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMultF32", _vp, _vp, _vp, _vp)
+_proto("dohpEllipShellMatMultAdd", _vp, _vp, _vp, _vp)
 _proto("dohpEllipSetScatter", _vp, ctypes.c_int)
 _proto("dohpEllipAssembleQ1", _vp, _vp, ctypes.POINTER(_vp), _vp)
 _proto("dohpMatGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long))
@@ -265,6 +266,10 @@ class Ellip:
 
     def mat_mult_f32(self, x, y, stream=None):
         check(_lib.dohpEllipShellMatMultF32(self.h, _ptr(x), _ptr(y), _stream(stream)))
+
+    def mat_mult_add(self, x, y, stream=None):
+        """y += J x: the element kernel alone (atomic scatter)."""
+        check(_lib.dohpEllipShellMatMultAdd(self.h, _ptr(x), _ptr(y), _stream(stream)))
 
     def assemble_q1(self, u=None, mat=None, stream=None):
         """Embedded-Q1 matrix at state u (EllipJacobian); reuses `mat` (same pattern) if given."""

@@ -22,28 +22,17 @@ namespace {
 
 enum { MODE_LIN = 0, MODE_FUN = 1, MODE_JAC = 2 };
 
-// Row stride of the LDS tables and element tiles: even, so rows start 16-byte aligned.
-constexpr int padded(int N) { return (N + 1) & ~1; }
-
 template <int N, typename T>
 struct Tab {
   double w[N];   // Gauss weights
   double phi[N]; // (1 + z_q)/2
 };
 
-// Basis tables staged in LDS once per workgroup, rows padded to NP entries:
-//   B[q][j] (GLL nodal basis at Gauss point q), Dh[q][r] (collocated derivative at the Gauss
-//   points), and their transposes, so that every row or column a thread needs is contiguous.
-template <int N, typename T>
-struct LdsTab {
-  static constexpr int NP = padded(N);
-  T B[N * NP], Dh[N * NP], BT[N * NP], DhT[N * NP];
-};
-
 struct Phys {
   double p, eps2, lambda, ea, eb, ec;
   int exact;
   int has_k;
+  int qpow;  // 4 (p-2)/2 when it is an integer in [-8, 8], else 99 (general power)
 };
 
 constexpr int elems_per_block(int N) { return (256 / (N * N)) > 0 ? 256 / (N * N) : 1; }
@@ -91,69 +80,6 @@ __device__ inline long xcd_block(long b, long nb)
   const long q = nb / 8, r = nb % 8, x = b % 8, pos = b / 8;
   return x < r ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
 }
-
-// Forward: column values col[] at the nodes -> uq (Gauss points) and reference gradient.
-// s1/s2 are this element's LDS tiles [i][j][k] with column stride NP.
-template <int N, typename T>
-__device__ inline void interp_grad(const LdsTab<N, T> &tb, const T (&col)[N], T (&uq)[N], T (&gx)[N], T (&gy)[N],
-                                   T (&gz)[N], T *s1, T *s2, int i, int j)
-{
-  constexpr int NP = padded(N);
-  T t[N], row[N];
-#pragma unroll
-  for (int q = 0; q < N; q++) {
-    T s = 0;
-#pragma unroll
-    for (int k = 0; k < N; k++) s += tb.B[q * NP + k] * col[k];
-    t[q] = s;
-  }
-#pragma unroll
-  for (int k = 0; k < N; k++) s1[(i * N + j) * NP + k] = t[k];
-#pragma unroll
-  for (int m = 0; m < N; m++) row[m] = tb.B[j * NP + m];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; k++) t[k] = 0;
-#pragma unroll
-  for (int m = 0; m < N; m++)
-#pragma unroll
-    for (int k = 0; k < N; k++) t[k] += row[m] * s1[(i * N + m) * NP + k];
-#pragma unroll
-  for (int k = 0; k < N; k++) s2[(i * N + j) * NP + k] = t[k];
-#pragma unroll
-  for (int m = 0; m < N; m++) row[m] = tb.B[i * NP + m];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; k++) uq[k] = 0;
-#pragma unroll
-  for (int m = 0; m < N; m++)
-#pragma unroll
-    for (int k = 0; k < N; k++) uq[k] += row[m] * s2[(m * N + j) * NP + k];
-#pragma unroll
-  for (int q = 0; q < N; q++) {
-    T s = 0;
-#pragma unroll
-    for (int k = 0; k < N; k++) s += tb.Dh[q * NP + k] * uq[k];
-    gz[q] = s;
-  }
-#pragma unroll
-  for (int k = 0; k < N; k++) s1[(i * N + j) * NP + k] = uq[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; k++) gy[k] = gx[k] = 0;
-#pragma unroll
-  for (int m = 0; m < N; m++) {
-    const T dj = tb.Dh[j * NP + m], di = tb.Dh[i * NP + m];
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-      gy[k] += dj * s1[(i * N + m) * NP + k];
-      gx[k] += di * s1[(m * N + j) * NP + k];
-    }
-  }
-}
-
-template <int N, typename T>
-__device__ inline T tb_l(const T *tab, int r, int c) { return tab[r * padded(N) + c]; }
 
 // Polynomial form of the column geometry: with c0, c1 linear in g = (1+z_k)/2 and c2 constant,
 // r0 = c1 x c2 and r1 = c2 x c0 are linear in g, r2 = c0 x c1 and det = c0 . r0 quadratic.  The
@@ -226,6 +152,215 @@ __device__ inline double recip(double d)
 }
 __device__ inline float recip(float d) { return __builtin_amdgcn_rcpf(d); }
 
+// ---------------------------------------------------------------------------------------------
+// 1-D operators in even-odd form.  The Gauss points and the GLL nodes are symmetric about 0, so
+// B[n-1-q][n-1-m] = B[q][m] and the collocated derivative has Dh[n-1-q][n-1-m] = -Dh[q][m] (the
+// same holds for the transposes).  With e[m] = in[m] + in[n-1-m] and o[m] = in[m] - in[n-1-m]
+// (m < h = n/2), rows q and n-1-q of out = A in share
+//     S = sum_m P[q][m] e[m] (+ Pm[q] in[h] for odd n),   D = sum_m M[q][m] o[m],
+//     out[q] = S + D,   out[n-1-q] = S - D (symmetric A) or D - S (antisymmetric A),
+// with P = (A[q][m] + A[q][n-1-m]) / 2 and M = (A[q][m] - A[q][n-1-m]) / 2; the middle row of odd
+// n is R . e (+ R[h] in[h]) for symmetric A and R . o for antisymmetric A.  Packed per operator:
+// [P: h*h][M: h*h][Pm: h][R: h+1] (the last two only for odd n).  The coefficients are uniform
+// over the workgroup and are read with scalar loads as instruction operands.
+constexpr int eo_size(int N) { return 2 * (N / 2) * (N / 2) + (N & 1) * (2 * (N / 2) + 1); }
+enum { OP_B = 0, OP_D = 1, OP_BT = 2, OP_DT = 3 };
+
+// Table loads issued after this point stay after it, so each contraction holds only its own
+// coefficients in scalar registers.
+template <typename T>
+__device__ __forceinline__ const T *sgpr_fence(const T *p)
+{
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+template <int N, int SYM, typename T>
+__device__ __forceinline__ void eo_apply(const T *__restrict__ tab, int op, const T (&in)[N], T (&out)[N])
+{
+  constexpr int H = N / 2, ODD = N & 1, HH = H * H;
+  typedef const __attribute__((address_space(4))) T ConstT;  // constant address space: scalar loads
+  const ConstT *A = (const ConstT *)(sgpr_fence(tab) + op * eo_size(N));
+  T e[H > 0 ? H : 1], o[H > 0 ? H : 1];
+#pragma unroll
+  for (int m = 0; m < H; m++) {
+    e[m] = in[m] + in[N - 1 - m];
+    o[m] = in[m] - in[N - 1 - m];
+  }
+#pragma unroll
+  for (int q = 0; q < H; q++) {
+    T sv = 0, dv = 0;
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+      sv = fma(A[q * H + m], e[m], sv);
+      dv = fma(A[HH + q * H + m], o[m], dv);
+    }
+    if (ODD) sv = fma(A[2 * HH + q], in[H], sv);
+    out[q] = sv + dv;
+    out[N - 1 - q] = SYM > 0 ? sv - dv : dv - sv;
+  }
+  if (ODD) {
+    T r = 0;
+#pragma unroll
+    for (int m = 0; m < H; m++) r = fma(A[2 * HH + H + m], SYM > 0 ? e[m] : o[m], r);
+    if (SYM > 0) r = fma(A[2 * HH + 2 * H], in[H], r);
+    out[H] = r;
+  }
+}
+
+// Element tiles [i][j][k] (k fastest, the reference's node order).  A thread (a, b) of an
+// element owns one line per phase: a z line (i, j) = (a, b), a y line (i, k) = (a, b) or an
+// x line (j, k) = (a, b).
+enum { DIR_X = 0, DIR_Y = 1, DIR_Z = 2 };
+template <int N, int DIR>
+__device__ __forceinline__ int line_at(int a, int b, int s)
+{
+  return DIR == DIR_Z ? (a * N + b) * N + s : DIR == DIR_Y ? (a * N + s) * N + b : (s * N + a) * N + b;
+}
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void ld_line(const T *tile, int a, int b, T (&v)[N])
+{
+#pragma unroll
+  for (int s = 0; s < N; s++) v[s] = tile[line_at<N, DIR>(a, b, s)];
+}
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void st_line(T *tile, int a, int b, const T (&v)[N])
+{
+#pragma unroll
+  for (int s = 0; s < N; s++) tile[line_at<N, DIR>(a, b, s)] = v[s];
+}
+
+// Interpolation and reference gradient of the nodal tile A at the Gauss points.  On return the
+// thread holds, along its z line (q1, q2) = (a, b): values q, gradients gx, gy, gz.  A is
+// overwritten; GX and GY are scratch.  Every thread of the workgroup must call it.
+template <int N, typename T>
+__device__ __forceinline__ void forward(const T *tab, T *A, T *GX, T *GY, int a, int b, T (&q)[N], T (&gx)[N],
+                                        T (&gy)[N], T (&gz)[N])
+{
+  T in[N], out[N];
+  ld_line<N, DIR_Z>(A, a, b, in);
+  eo_apply<N, 1>(tab, OP_B, in, out);
+  st_line<N, DIR_Z>(A, a, b, out);
+  __syncthreads();
+  ld_line<N, DIR_Y>(A, a, b, in);
+  eo_apply<N, 1>(tab, OP_B, in, out);
+  st_line<N, DIR_Y>(A, a, b, out);
+  __syncthreads();
+  ld_line<N, DIR_X>(A, a, b, in);
+  eo_apply<N, 1>(tab, OP_B, in, out);
+  st_line<N, DIR_X>(A, a, b, out);
+  eo_apply<N, -1>(tab, OP_D, out, in);
+  st_line<N, DIR_X>(GX, a, b, in);
+  __syncthreads();
+  ld_line<N, DIR_Y>(A, a, b, in);
+  eo_apply<N, -1>(tab, OP_D, in, out);
+  st_line<N, DIR_Y>(GY, a, b, out);
+  __syncthreads();
+  ld_line<N, DIR_Z>(A, a, b, q);
+  eo_apply<N, -1>(tab, OP_D, q, gz);
+  ld_line<N, DIR_Z>(GX, a, b, gx);
+  ld_line<N, DIR_Z>(GY, a, b, gy);
+}
+
+// Transposed applies: from the pointwise values v and the mapped fluxes d0, d1, d2 along the
+// thread's z line, the element's nodal contributions B^T (v + Dh_x^T d0 + Dh_y^T d1 + Dh_z^T d2)
+// land in tile A.  Ends with a barrier.
+template <int N, typename T>
+__device__ __forceinline__ void backward(const T *tab, T *A, T *GX, T *GY, int a, int b, const T (&v)[N],
+                                         const T (&d0)[N], const T (&d1)[N], const T (&d2)[N])
+{
+  T w[N], in[N], out[N];
+  eo_apply<N, -1>(tab, OP_DT, d2, w);
+#pragma unroll
+  for (int s = 0; s < N; s++) w[s] += v[s];
+  st_line<N, DIR_Z>(A, a, b, w);
+  st_line<N, DIR_Z>(GY, a, b, d1);
+  st_line<N, DIR_Z>(GX, a, b, d0);
+  __syncthreads();
+  ld_line<N, DIR_Y>(GY, a, b, in);
+  eo_apply<N, -1>(tab, OP_DT, in, out);
+  ld_line<N, DIR_Y>(A, a, b, w);
+#pragma unroll
+  for (int s = 0; s < N; s++) w[s] += out[s];
+  st_line<N, DIR_Y>(A, a, b, w);
+  __syncthreads();
+  ld_line<N, DIR_X>(GX, a, b, in);
+  eo_apply<N, -1>(tab, OP_DT, in, out);
+  ld_line<N, DIR_X>(A, a, b, w);
+#pragma unroll
+  for (int s = 0; s < N; s++) w[s] += out[s];
+  eo_apply<N, 1>(tab, OP_BT, w, out);
+  st_line<N, DIR_X>(A, a, b, out);
+  __syncthreads();
+  ld_line<N, DIR_Y>(A, a, b, in);
+  eo_apply<N, 1>(tab, OP_BT, in, out);
+  st_line<N, DIR_Y>(A, a, b, out);
+  __syncthreads();
+  ld_line<N, DIR_Z>(A, a, b, in);
+  eo_apply<N, 1>(tab, OP_BT, in, out);
+  st_line<N, DIR_Z>(A, a, b, out);
+  __syncthreads();
+}
+
+// x^e for x > 0.  Quarter-integer exponents (ph.qpow = 4e in [-8, 8]) use the hardware
+// reciprocal square root refined by Newton steps; other exponents call pow.
+__device__ inline double rsqrt_nr(double x)
+{
+  double y = __builtin_amdgcn_rsq(x);
+  double r = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, r, y);
+  r = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, r, y);
+}
+__device__ inline float rsqrt_nr(float x) { return __builtin_amdgcn_rsqf(x); }
+
+template <typename T>
+__device__ inline T powe(T x, T e, int qpow)
+{
+  if (qpow == 0) return T(1);
+  if (qpow > 8 || qpow < -8) return pow(x, e);
+  const T r2 = rsqrt_nr(x);         // x^(-1/2)
+  const T r4 = r2 * rsqrt_nr(r2);   // x^(-1/4)
+  const int n = qpow < 0 ? -qpow : qpow;
+  const T base = qpow < 0 ? r4 : T(1) / r4;
+  T acc = T(1);
+#pragma unroll 1
+  for (int t = 0; t < n; t++) acc *= base;
+  return acc;
+}
+
+// sqrt for the rank-one factor: negative arguments (p > 2) give NaN as sqrt does.
+template <typename T>
+__device__ inline T sqrt_pos(T x) { return x > T(0) ? x * rsqrt_nr(x) : sqrt(x); }
+
+// Column geometry of the trilinear map along the z line (xi0 = z_a, xi1 = z_b) of an element
+// whose 8 vertices (x, y, z, k) sit at X[32]; also the line's coordinate and coefficient ends.
+template <int N, typename T>
+__device__ __forceinline__ ColGeo<T> column_geometry(const double *X, const Tab<N, T> &tb, int a, int b,
+                                                     double (&Pc)[2][3], double (&Kc)[2])
+{
+  const double fa1 = tb.phi[a], fa0 = 1.0 - fa1, fb1 = tb.phi[b], fb0 = 1.0 - fb1;
+  double Rc[2][3], Sc[2][3];
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const double x00 = X[(0 + c) * 4 + d], x01 = X[(2 + c) * 4 + d], x10 = X[(4 + c) * 4 + d],
+                   x11 = X[(6 + c) * 4 + d];
+      const double pc = fa0 * (fb0 * x00 + fb1 * x01) + fa1 * (fb0 * x10 + fb1 * x11);
+      if (d < 3) {
+        Pc[c][d] = pc;
+        Rc[c][d] = 0.5 * (fb0 * (x10 - x00) + fb1 * (x11 - x01));
+        Sc[c][d] = 0.5 * (fa0 * (x01 - x00) + fa1 * (x11 - x10));
+      } else {
+        Kc[c] = pc;
+      }
+    }
+  }
+  const double c2[3] = {0.5 * (Pc[1][0] - Pc[0][0]), 0.5 * (Pc[1][1] - Pc[0][1]), 0.5 * (Pc[1][2] - Pc[0][2])};
+  return make_colgeo<T>(Rc, Sc, c2);
+}
+
 template <int N, typename T, int MODE>
 __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elist, long ne,
                                                     const int *__restrict__ enodes, const int *__restrict__ everts,
@@ -234,19 +369,17 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
                                                     T *__restrict__ y, long nowned, int atomic,
                                                     const T *__restrict__ gtab, Tab<N, T> tb, Phys ph)
 {
-  constexpr int NN = N * N, N3 = NN * N, EPB = elems_per_block(N), NP = padded(N), CS = NN * NP;
-  constexpr int N3X = MODE == MODE_LIN ? 1 : EPB * CS;
-  __shared__ LdsTab<N, T> TB;
-  __shared__ T S1[EPB * CS], S2[EPB * CS], S3[N3X];
+  constexpr int NN = N * N, N3 = NN * N, EPB = elems_per_block(N), NTH = EPB * NN;
+  __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
   __shared__ int IDX[EPB * N3];
   __shared__ double VX[EPB * 8 * 4];
 
   const long lb = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
-  const int le = tid / NN, ij = tid - le * NN, i = ij / N, j = ij - i * N;
-  for (int t = tid; t < 4 * N * NP; t += blockDim.x) (&TB.B[0])[t] = gtab[t];
-  for (int t = tid; t < EPB * N3; t += blockDim.x) {
-    const int l = t / N3;
+  const int le = tid / NN, ab = tid - le * NN, a = ab / N, b = ab - a * N;
+#pragma unroll
+  for (int s = 0; s < N; s++) {
+    const int t = s * NTH + tid, l = t / N3;
     const long es = lb * EPB + l;
     int v = -1;
     if (es < ne) {
@@ -255,7 +388,7 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
     }
     IDX[t] = v;
   }
-  for (int t = tid; t < EPB * 8; t += blockDim.x) {
+  for (int t = tid; t < EPB * 8; t += NTH) {
     const int l = t >> 3, vt = t & 7;
     const long es = lb * EPB + l;
     double *dst = &VX[(l * 8 + vt) * 4];
@@ -274,58 +407,29 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
     }
   }
   __syncthreads();
-  // gather (coalesced over the element's node list)
-  for (int t = tid; t < EPB * N3; t += blockDim.x) {
-    const int c = IDX[t];
-    T a = 0, b = 0;
-    if (c >= 0) {
-      if (MODE == MODE_FUN) a = ustate[c];
-      if (MODE == MODE_JAC) { a = ustate[c]; b = c < nowned ? xdir[c] : T(0); }
-      if (MODE == MODE_LIN) a = c < nowned ? xdir[c] : T(0);
-    }
-    const int l = t / N3, r = t - l * N3, col = r / N, k = r - col * N;
-    S1[l * CS + col * NP + k] = a;
-    if (MODE == MODE_JAC) S3[l * CS + col * NP + k] = b;
+  // gather of the first field, coalesced over the element node lists (tile order = node order);
+  // the Jacobian action prefetches the state into registers behind the direction's contractions
+  T upre[MODE == MODE_JAC ? N : 1];
+#pragma unroll
+  for (int s = 0; s < N; s++) {
+    const int t = s * NTH + tid, c = IDX[t];
+    T v = 0;
+    if (MODE == MODE_FUN) v = c >= 0 ? ustate[c] : T(0);
+    else v = (c >= 0 && c < nowned) ? xdir[c] : T(0);
+    TA[t] = v;
+    if (MODE == MODE_JAC) upre[s] = c >= 0 ? ustate[c] : T(0);
   }
   __syncthreads();
-  T *s1 = &S1[le * CS], *s2 = &S2[le * CS];
-  T ucol[N], xcol[N];
-#pragma unroll
-  for (int k = 0; k < N; k++) {
-    ucol[k] = s1[(i * N + j) * NP + k];
-    if (MODE == MODE_JAC) xcol[k] = S3[le * CS + (i * N + j) * NP + k];
-  }
-  __syncthreads();
+  T *A = &TA[le * N3], *GX = &TGX[le * N3], *GY = &TGY[le * N3];
 
-  // geometry of the trilinear map along this thread's column (xi0 = z_i, xi1 = z_j)
-  const double *X = &VX[le * 32];
-  const double fa1 = tb.phi[i], fa0 = 1.0 - fa1, fb1 = tb.phi[j], fb0 = 1.0 - fb1;
-  double Pc[2][3], Rc[2][3], Sc[2][3], Kc[2];
-#pragma unroll
-  for (int c = 0; c < 2; c++) {
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-      const double x00 = X[(0 + c) * 4 + d], x01 = X[(2 + c) * 4 + d], x10 = X[(4 + c) * 4 + d],
-                   x11 = X[(6 + c) * 4 + d];
-      const double pc = fa0 * (fb0 * x00 + fb1 * x01) + fa1 * (fb0 * x10 + fb1 * x11);
-      if (d < 3) {
-        Pc[c][d] = pc;
-        Rc[c][d] = 0.5 * (fb0 * (x10 - x00) + fb1 * (x11 - x01));
-        Sc[c][d] = 0.5 * (fa0 * (x01 - x00) + fa1 * (x11 - x10));
-      } else {
-        Kc[c] = pc;
-      }
-    }
-  }
-  const double c2[3] = {0.5 * (Pc[1][0] - Pc[0][0]), 0.5 * (Pc[1][1] - Pc[0][1]), 0.5 * (Pc[1][2] - Pc[0][2])};
-  const ColGeo<T> cg = make_colgeo<T>(Rc, Sc, c2);
-  const T wij = (T)(tb.w[i] * tb.w[j]);
-
-  T uq[N], gx[N], gy[N], gz[N];
-  interp_grad<N, T>(TB, ucol, uq, gx, gy, gz, s1, s2, i, j);
-
+  // geometry is formed after the forward contractions so that it does not occupy registers there
+  const T wij = (T)(tb.w[a] * tb.w[b]);
+  double Pc[2][3], Kc[2];
   T vq[N], d0[N], d1[N], d2[N];
   if (MODE == MODE_LIN) {
+    T xq[N], hx[N], hy[N], hz[N];
+    forward<N, T>(gtab, A, GX, GY, a, b, xq, hx, hy, hz);
+    const ColGeo<T> cg = column_geometry<N, T>(&VX[le * 32], tb, a, b, Pc, Kc);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       T r0[3], r1[3], r2[3], det;
@@ -333,7 +437,7 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
       const T kq = ph.has_k ? (T)(Kc[0] + (Kc[1] - Kc[0]) * tb.phi[k]) : T(1);
       // Dv = jw k grad(x) = w k (sum_m ghat_m r_m) / det;  df_m = jinv[m][:] . Dv = r_m . Dv / det
       const T sc = wij * (T)tb.w[k] * kq * recip(det);
-      const T a0 = gx[k], a1 = gy[k], a2 = gz[k];
+      const T a0 = hx[k], a1 = hy[k], a2 = hz[k];
       T s3[3];
 #pragma unroll
       for (int d = 0; d < 3; d++) s3[d] = a0 * r0[d] + a1 * r1[d] + a2 * r2[d];
@@ -343,19 +447,28 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
       d2[k] = sc * (r2[0] * s3[0] + r2[1] * s3[1] + r2[2] * s3[2]);
     }
   } else {
+    T xq[N], hx[N], hy[N], hz[N];
+    if constexpr (MODE == MODE_JAC) {
+      forward<N, T>(gtab, A, GX, GY, a, b, xq, hx, hy, hz);
+      __syncthreads();  // every line read before the state overwrites the tile
+#pragma unroll
+      for (int s = 0; s < N; s++) TA[s * NTH + tid] = upre[s];
+      __syncthreads();
+    }
+    T uq[N], gu0[N], gu1[N], gu2[N];
+    forward<N, T>(gtab, A, GX, GY, a, b, uq, gu0, gu1, gu2);
+    const ColGeo<T> cg = column_geometry<N, T>(&VX[le * 32], tb, a, b, Pc, Kc);
     // Pointwise physics of the state in three phases so that one copy of the transcendental
-    // code is emitted: (a) |grad u|^2 and jw per point into this thread's own LDS slots,
+    // code is emitted: (a) |grad u|^2 and jw per point into this thread's own line slots,
     // (b) eta, the rank-one Jacobian factor, forcing and Bratu term in a rolled loop, (c) the
-    // per-point store.  Own-slot LDS traffic needs no barrier.
-    __syncthreads();  // S1/S2/S3 are free once every column is in registers
-    T *E = &S1[le * CS + (i * N + j) * NP], *F1 = &S2[le * CS + (i * N + j) * NP], *U = &S3[le * CS + (i * N + j) * NP];
-    T gu0[N], gu1[N], gu2[N];
+    // per-point fluxes.  Each thread touches only its own z lines here: no barrier.
+    T *E = &A[(a * N + b) * N], *F1 = &GX[(a * N + b) * N], *U = &GY[(a * N + b) * N];
 #pragma unroll
     for (int k = 0; k < N; k++) {
       T r0[3], r1[3], r2[3], det;
       cg.eval((T)tb.phi[k], r0, r1, r2, det);
       const T idet = recip(det);
-      const T a0 = gx[k], a1 = gy[k], a2 = gz[k];
+      const T a0 = gu0[k], a1 = gu1[k], a2 = gu2[k];
       gu0[k] = (a0 * r0[0] + a1 * r1[0] + a2 * r2[0]) * idet;
       gu1[k] = (a0 * r0[1] + a1 * r1[1] + a2 * r2[1]) * idet;
       gu2[k] = (a0 * r0[2] + a1 * r1[2] + a2 * r2[2]) * idet;
@@ -363,11 +476,12 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
       F1[k] = wij * (T)tb.w[k] * det;  // jw
       U[k] = uq[k];
     }
+    const T ex = T((ph.p - 2) / 2);
 #pragma unroll 1
     for (int k = 0; k < N; k++) {
       const T espg = E[k], jw = F1[k];
       const T kq = ph.has_k ? (T)(Kc[0] + (Kc[1] - Kc[0]) * tb.phi[k]) : T(1);
-      const T eta = pow(espg, T((ph.p - 2) / 2));
+      const T eta = powe(espg, ex, ph.qpow);
       const T lexp = ph.lambda != 0 ? T(ph.lambda) * exp(U[k]) : T(0);
       if (MODE == MODE_FUN) {
         double f = 0.0;
@@ -381,124 +495,47 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
         // Dv = jw k (eta grad x - (s . grad x) s), s = sqrt(-(p-2)/2 eta / espg) grad u; folded:
         // Dv = E grad x - (sv . grad x) sv with sv = sqrt(jw k) s
         E[k] = jw * eta * kq;
-        F1[k] = sqrt(-T((ph.p - 2) / 2) * eta / espg * jw * kq);
+        F1[k] = sqrt_pos(-ex * eta / espg * jw * kq);
         U[k] = -jw * lexp;
       }
     }
-    if (MODE == MODE_FUN) {
 #pragma unroll
-      for (int k = 0; k < N; k++) {
-        T r0[3], r1[3], r2[3], det;
-        cg.eval((T)tb.phi[k], r0, r1, r2, det);
+    for (int k = 0; k < N; k++) {
+      T r0[3], r1[3], r2[3], det;
+      cg.eval((T)tb.phi[k], r0, r1, r2, det);
+      const T idet = recip(det);
+      T Dv[3];
+      if (MODE == MODE_FUN) {
         const T sc = E[k];
-        const T Dv[3] = {sc * gu0[k], sc * gu1[k], sc * gu2[k]};
-        const T idet = recip(det);
+        Dv[0] = sc * gu0[k];
+        Dv[1] = sc * gu1[k];
+        Dv[2] = sc * gu2[k];
         vq[k] = U[k];
-        d0[k] = (r0[0] * Dv[0] + r0[1] * Dv[1] + r0[2] * Dv[2]) * idet;
-        d1[k] = (r1[0] * Dv[0] + r1[1] * Dv[1] + r1[2] * Dv[2]) * idet;
-        d2[k] = (r2[0] * Dv[0] + r2[1] * Dv[1] + r2[2] * Dv[2]) * idet;
-      }
-    } else {
-      // store: etaw, sv (3), lamjw per point; the state's fields are dead after this
-      T etaw[N], sv0[N], sv1[N], sv2[N], lam[N];
-#pragma unroll
-      for (int k = 0; k < N; k++) {
-        const T c1 = F1[k];
-        etaw[k] = E[k];
-        sv0[k] = c1 * gu0[k];
-        sv1[k] = c1 * gu1[k];
-        sv2[k] = c1 * gu2[k];
-        lam[k] = U[k];
-      }
-      __syncthreads();
-      T xq[N], hx[N], hy[N], hz[N];
-      interp_grad<N, T>(TB, xcol, xq, hx, hy, hz, s1, s2, i, j);
-#pragma unroll
-      for (int k = 0; k < N; k++) {
-        T r0[3], r1[3], r2[3], det;
-        cg.eval((T)tb.phi[k], r0, r1, r2, det);
-        const T idet = recip(det);
+      } else {
         const T b0 = hx[k], b1 = hy[k], b2 = hz[k];
         T gxd[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) gxd[d] = (b0 * r0[d] + b1 * r1[d] + b2 * r2[d]) * idet;
-        const T dot = sv0[k] * gxd[0] + sv1[k] * gxd[1] + sv2[k] * gxd[2];
-        const T Dv[3] = {etaw[k] * gxd[0] - dot * sv0[k], etaw[k] * gxd[1] - dot * sv1[k],
-                         etaw[k] * gxd[2] - dot * sv2[k]};
-        vq[k] = lam[k] * xq[k];
-        d0[k] = (r0[0] * Dv[0] + r0[1] * Dv[1] + r0[2] * Dv[2]) * idet;
-        d1[k] = (r1[0] * Dv[0] + r1[1] * Dv[1] + r1[2] * Dv[2]) * idet;
-        d2[k] = (r2[0] * Dv[0] + r2[1] * Dv[1] + r2[2] * Dv[2]) * idet;
+        const T c1 = F1[k], etaw = E[k];
+        const T sv0 = c1 * gu0[k], sv1 = c1 * gu1[k], sv2 = c1 * gu2[k];
+        const T dot = sv0 * gxd[0] + sv1 * gxd[1] + sv2 * gxd[2];
+        Dv[0] = etaw * gxd[0] - dot * sv0;
+        Dv[1] = etaw * gxd[1] - dot * sv1;
+        Dv[2] = etaw * gxd[2] - dot * sv2;
+        vq[k] = U[k] * xq[k];
       }
+      d0[k] = (r0[0] * Dv[0] + r0[1] * Dv[1] + r0[2] * Dv[2]) * idet;
+      d1[k] = (r1[0] * Dv[0] + r1[1] * Dv[1] + r1[2] * Dv[2]) * idet;
+      d2[k] = (r2[0] * Dv[0] + r2[1] * Dv[1] + r2[2] * Dv[2]) * idet;
     }
   }
-
-  // transposed collocated derivative, combined at the Gauss points
-  T t[N], row[N], rw2[N];
+  backward<N, T>(gtab, A, GX, GY, a, b, vq, d0, d1, d2);
+  // scatter (coalesced over the element node lists); Dirichlet rows are discarded
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    T s = vq[k];
-#pragma unroll
-    for (int m = 0; m < N; m++) s += tb_l<N, T>(TB.DhT, k, m) * d2[m];
-    t[k] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; k++) {
-    s1[(i * N + j) * NP + k] = d1[k];
-    s2[(i * N + j) * NP + k] = d0[k];
-  }
-#pragma unroll
-  for (int m = 0; m < N; m++) {
-    row[m] = TB.DhT[j * NP + m];
-    rw2[m] = TB.DhT[i * NP + m];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < N; m++)
-#pragma unroll
-    for (int k = 0; k < N; k++) t[k] += row[m] * s1[(i * N + m) * NP + k] + rw2[m] * s2[(m * N + j) * NP + k];
-  // B^T along z, y, x
-  T t2[N];
-#pragma unroll
-  for (int k = 0; k < N; k++) {
-    T s = 0;
-#pragma unroll
-    for (int q = 0; q < N; q++) s += tb_l<N, T>(TB.BT, k, q) * t[q];
-    t2[k] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; k++) s1[(i * N + j) * NP + k] = t2[k];
-#pragma unroll
-  for (int m = 0; m < N; m++) row[m] = TB.BT[j * NP + m];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; k++) t2[k] = 0;
-#pragma unroll
-  for (int m = 0; m < N; m++)
-#pragma unroll
-    for (int k = 0; k < N; k++) t2[k] += row[m] * s1[(i * N + m) * NP + k];
-#pragma unroll
-  for (int k = 0; k < N; k++) s2[(i * N + j) * NP + k] = t2[k];
-#pragma unroll
-  for (int m = 0; m < N; m++) row[m] = TB.BT[i * NP + m];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; k++) t[k] = 0;
-#pragma unroll
-  for (int m = 0; m < N; m++)
-#pragma unroll
-    for (int k = 0; k < N; k++) t[k] += row[m] * s2[(m * N + j) * NP + k];
-#pragma unroll
-  for (int k = 0; k < N; k++) s1[(i * N + j) * NP + k] = t[k];
-  __syncthreads();
-  // scatter (coalesced over the element's node list); Dirichlet rows are discarded
-  for (int tt = tid; tt < EPB * N3; tt += blockDim.x) {
-    const int c = IDX[tt];
+  for (int s = 0; s < N; s++) {
+    const int t = s * NTH + tid, c = IDX[t];
     if (c >= 0 && c < nowned) {
-      const int l = tt / N3, r = tt - l * N3, col = r / N, k = r - col * N;
-      const T val = S1[l * CS + col * NP + k];
+      const T val = TA[t];
       if (atomic) atomicAdd(&y[c], val);
       else y[c] += val;
     }
@@ -601,6 +638,8 @@ Phys make_phys(const _p_dohpEllip *op)
   ph.ec = op->prm.c;
   ph.exact = op->prm.exact;
   ph.has_k = op->kvert.ptr != nullptr;
+  const double q4 = 2.0 * (op->prm.p - 2.0);
+  ph.qpow = (q4 == std::floor(q4) && std::fabs(q4) <= 8.0) ? (int)q4 : 99;
   return ph;
 }
 
@@ -699,16 +738,27 @@ extern "C" dErr dohpEllipCreate(dohpFS fs, const dohpEllipParam *prm, dohpEllip 
         err = dohpERR_NEGATIVE_JACOBIAN;
       }
   }
-  {  // B, Dh, B^T, Dh^T with rows padded to an even length
-    const int NP = (N + 1) & ~1;
-    std::vector<double> t(4 * N * NP, 0.0);
-    for (int q = 0; q < N; q++)
-      for (int r = 0; r < N; r++) {
-        t[0 * N * NP + q * NP + r] = op->B[q * N + r];
-        t[1 * N * NP + q * NP + r] = op->Dh[q * N + r];
-        t[2 * N * NP + r * NP + q] = op->B[q * N + r];
-        t[3 * N * NP + r * NP + q] = op->Dh[q * N + r];
+  {  // B, Dh, B^T, Dh^T in even-odd form (see eo_apply)
+    const int H = N / 2, S = eo_size(N);
+    std::vector<double> t(4 * S, 0.0);
+    for (int opi = 0; opi < 4; opi++) {
+      auto A = [&](int q, int m) {
+        const std::vector<double> &M = (opi == OP_B || opi == OP_BT) ? op->B : op->Dh;
+        return (opi == OP_B || opi == OP_D) ? M[q * N + m] : M[m * N + q];
+      };
+      const bool sym = opi == OP_B || opi == OP_BT;
+      double *T0 = &t[opi * S];
+      for (int q = 0; q < H; q++)
+        for (int m = 0; m < H; m++) {
+          T0[q * H + m] = 0.5 * (A(q, m) + A(q, N - 1 - m));
+          T0[H * H + q * H + m] = 0.5 * (A(q, m) - A(q, N - 1 - m));
+        }
+      if (N & 1) {
+        for (int q = 0; q < H; q++) T0[2 * H * H + q] = A(q, H);
+        for (int m = 0; m < H; m++) T0[2 * H * H + H + m] = A(H, m);
+        T0[2 * H * H + 2 * H] = sym ? A(H, H) : 0.0;
       }
+    }
     std::vector<float> tf(t.begin(), t.end());
     if (!err) err = op->tab64.upload(t.data(), t.size());
     if (!err) err = op->tab32.upload(tf.data(), tf.size());
@@ -789,6 +839,16 @@ extern "C" dErr dohpEllipShellMatMult(dohpEllip op, const double *x, double *y, 
   DOHP_CHECK(op && x && y, dohpERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
   DOHP_CALL(zero(y, op->fs->nowned, st));
+  if (is_linear(op)) return launch<double, MODE_LIN>(op, nullptr, x, y, st);
+  DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
+  return launch<double, MODE_JAC>(op, op->state.ptr, x, y, st);
+}
+
+extern "C" dErr dohpEllipShellMatMultAdd(dohpEllip op, const double *x, double *y, void *stream)
+{
+  DOHP_CHECK(op && x && y, dohpERR_ARG, "null argument");
+  DOHP_CHECK(op->scatter == 1, dohpERR_ARG, "MatMultAdd accumulates with the atomic scatter only");
+  hipStream_t st = (hipStream_t)stream;
   if (is_linear(op)) return launch<double, MODE_LIN>(op, nullptr, x, y, st);
   DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
   return launch<double, MODE_JAC>(op, op->state.ptr, x, y, st);

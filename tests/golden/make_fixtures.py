@@ -90,7 +90,12 @@ def jacobi4_patches(path):
 def ellip_history(path):
     """SNES function norms and KSP residual histories of an ellip run (src/fs/tests/ellip.c)."""
     snes, ksp, cur = [], [], None
+    norms = {}
     for line in open(path):
+        m = re.match(r"Pointwise (solution|gradient) error\s+\|x\|_1 (\S+)\s+\|x\|_2 (\S+)\s+\|x\|_inf (\S+)", line)
+        if m:
+            norms[m.group(1)] = [float(m.group(k)) for k in (2, 3, 4)]
+            continue
         m = re.match(r"\s*(\d+) SNES Function norm (<)?\s*([\d.e+-]+)", line)
         if m:
             snes.append(float(m.group(3)) if not m.group(2) else 0.0)
@@ -103,7 +108,10 @@ def ellip_history(path):
                 cur = []
                 ksp.append(cur)
             cur.append(float(m.group(3)) if not m.group(2) else 0.0)
-    return {"snes": snes, "ksp": [k for k in ksp if k]}
+    out = {"snes": snes, "ksp": [k for k in ksp if k]}
+    if norms:
+        out["error_norms"] = norms
+    return out
 
 
 def main(ref):

@@ -273,3 +273,51 @@ def test_gather_scatter_sum_invariant():
     cnt = np.bincount(S.elem_nodes.reshape(-1), minlength=S.nclosure)
     assert cnt.sum() == S.nelem * 125
     assert cnt.max() == 8 and cnt.min() == 1
+
+
+NORM_CASES = {"ellip-e0-b1-p16": CASES["ellip-e0-b1-p16"], "ellip-e0-b3-p16": CASES["ellip-e0-b3-p16"],
+              "ellip-e0-b3-p14-sse": CASES["ellip-e0-b3-p14-sse"],
+              "ellip-e0-b3-p17-lam1-sse": CASES["ellip-e0-b3-p17-lam1-sse"]}
+
+
+@pytest.mark.parametrize("name", list(NORM_CASES))
+def test_ellip_converged_error_norms(name):
+    """Newton to the discrete solution, then the printed pointwise solution/gradient error norms
+    (util.c:110-168 through ellip.c:538-564), three significant digits in the refout."""
+    S, E, d = _ellip(*NORM_CASES[name])
+    nd = len(d)
+    x = np.zeros(S.nowned)
+    fn = lambda x: E.function(E.closure(x, d))[:S.nowned]
+    F = fn(x)
+    for it in range(40):
+        if np.linalg.norm(F) < 1e-11:
+            break
+        uc = E.closure(x, d)
+        lu = spla.splu(E.q1_matrix(uc).tocsc())
+        A = lambda v, uc=uc: E.jacobian_apply(uc, np.concatenate([v, np.zeros(nd)]))[:S.nowned]
+        dx, _ = _gmres_left(A, lu.solve, F, 1e-12, maxit=400)
+        lam = 1.0  # backtracking on ||F|| (the discrete solution does not depend on the path)
+        while True:
+            Fn = fn(x - lam * dx)
+            if np.linalg.norm(Fn) <= (1 - 1e-4 * lam) * np.linalg.norm(F) or lam < 1e-3:
+                break
+            lam /= 2
+        x, F = x - lam * dx, Fn
+    assert np.linalg.norm(F) < 1e-10
+    en, gn = E.error_norms(E.closure(x, d))
+    ref = HIST[name]["error_norms"]
+    np.testing.assert_allclose(en, ref["solution"], rtol=5e-3)
+    np.testing.assert_allclose(gn, ref["gradient"], rtol=5e-3)
+
+
+def test_ellip_morph_linear_solve_error_norms():
+    S, E, d = _ellip(3, 2.0, 0.0, (1, 1, 1), morph=1.0, twist=0.4)
+    uc = E.closure(np.zeros(S.nowned), d)
+    F = E.function(uc)[:S.nowned]
+    lu = spla.splu(E.q1_matrix(uc).tocsc())
+    A = lambda v: E.jacobian_apply(uc, np.concatenate([v, np.zeros(len(d))]))[:S.nowned]
+    dx, _ = _gmres_left(A, lu.solve, F, 1e-12, maxit=400)
+    en, gn = E.error_norms(E.closure(-dx, d))
+    ref = HIST["ellip-morph"]["error_norms"]
+    np.testing.assert_allclose(en, ref["solution"], rtol=5e-3)
+    np.testing.assert_allclose(gn, ref["gradient"], rtol=5e-3)

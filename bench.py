@@ -16,7 +16,7 @@ profile summary committed under profiles/ when it covers this workload.
 
 --workload solve: variable-coefficient Poisson -div(k grad u) = f with log-normal k (unit
 variance, correlation length 0.1, seeded) on 48^3 elements, order 5; CG preconditioned by the
-embedded-Q1 matrix under Jacobi-Chebyshev, plain FP64 versus FP32-inner/FP64-outer defect
+embedded-Q1 matrix (multigrid V-cycle on it with Jacobi-Chebyshev smoothing), plain FP64 versus FP32-inner/FP64-outer defect
 correction, both to relative residual 1e-10; prints one JSON line with iterations, times and
 the final FP64 error against the manufactured solution.
 
@@ -154,7 +154,7 @@ def solve_workload(args):
         print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
     out = {"metric": "solve time to relative residual 1e-10", "value": res["mixed_f32_f64"]["seconds"], "unit": "s",
            "n_gpus": 1, "higher_is_better": False, "dtype": "f32 inner / f64 outer", "data": "synthetic",
-           "config": {"workload": f"log-normal k Poisson, {Me}^3 hex, p={p}, CG + embedded-Q1 Jacobi-Chebyshev",
+           "config": {"workload": f"log-normal k Poisson, {Me}^3 hex, p={p}, CG + multigrid on the embedded Q1 (Jacobi-Chebyshev smoothing)",
                       "n_int": fs.nowned, "q1_nnz": q1.nnz, "q1_assembly_ms": round(t_asm * 1e3, 2)},
            "speedup_mixed_vs_plain": round(res["plain_f64"]["seconds"] / res["mixed_f32_f64"]["seconds"], 3),
            "runs": res}

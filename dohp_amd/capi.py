@@ -73,10 +73,40 @@ _proto("dohpMatMult", _vp, _vp, _vp, _vp)
 _proto("dohpMatMultF32", _vp, _vp, _vp, _vp)
 _proto("dohpMatGetDiagonal", _vp, _vp, _vp)
 _proto("dohpMatDestroy", ctypes.POINTER(_vp))
+class MG:
+    """Multigrid V-cycle on the embedded-Q1 matrix (dohpMG)."""
+
+    def __init__(self, q1, smooth_degree=2, max_coarse=0, stream=None):
+        self.h = ctypes.c_void_p()
+        check(_lib.dohpMGCreate(q1.h, int(smooth_degree), int(max_coarse), ctypes.byref(self.h), _stream(stream)))
+        nl, nc, lm = ctypes.c_int(), ctypes.c_long(), ctypes.c_double()
+        check(_lib.dohpMGGetInfo(self.h, ctypes.byref(nl), ctypes.byref(nc), ctypes.byref(lm)))
+        self.nlevels, self.ncoarse, self.lmax = nl.value, nc.value, lm.value
+
+    def level(self, l):
+        box = (ctypes.c_int * 3)()
+        check(_lib.dohpMGGetLevel(self.h, int(l), box, None))
+        n = box[0] * box[1] * box[2]
+        st = np.zeros(27 * n)
+        check(_lib.dohpMGGetLevel(self.h, int(l), box, st.ctypes.data_as(ctypes.c_void_p)))
+        return tuple(box), st.reshape(27, n)
+
+    def apply(self, r, z, stream=None):
+        check(_lib.dohpMGApply(self.h, _ptr(r), _ptr(z), _stream(stream)))
+
+    def apply_f32(self, r, z, stream=None):
+        check(_lib.dohpMGApplyF32(self.h, _ptr(r), _ptr(z), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.dohpMGDestroy(ctypes.byref(self.h))
+
+
 class SolveOpts(ctypes.Structure):
     _fields_ = [("mixed", ctypes.c_int), ("rtol", ctypes.c_double), ("max_it", ctypes.c_int),
                 ("inner_rtol", ctypes.c_double), ("max_outer", ctypes.c_int), ("cheb_degree", ctypes.c_int),
-                ("cheb_ratio", ctypes.c_double), ("eig_its", ctypes.c_int)]
+                ("cheb_ratio", ctypes.c_double), ("eig_its", ctypes.c_int), ("pc", ctypes.c_int),
+                ("mg_degree", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -85,6 +115,12 @@ class SolveStats(ctypes.Structure):
                 ("seconds", ctypes.c_double)]
 
 
+_proto("dohpMGCreate", _vp, ctypes.c_int, ctypes.c_int, _vp, _vp)
+_proto("dohpMGGetInfo", _vp, _vp, _vp, _vp)
+_proto("dohpMGGetLevel", _vp, ctypes.c_int, _vp, _vp)
+_proto("dohpMGApply", _vp, _vp, _vp, _vp)
+_proto("dohpMGApplyF32", _vp, _vp, _vp, _vp)
+_proto("dohpMGDestroy", _vp)
 _proto("dohpEllipSolve", _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)
 _lib.dohpGetLastError.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 _lib.dohpGetLastError.restype = None
@@ -279,10 +315,11 @@ class Ellip:
         return m
 
     def solve(self, q1, b, x, mixed=False, rtol=1e-10, max_it=2000, inner_rtol=1e-4, max_outer=50, cheb_degree=4,
-              cheb_ratio=30.0, eig_its=20, stream=None):
-        """CG on the linear operator, preconditioned by Jacobi-Chebyshev on the embedded-Q1 matrix."""
+              cheb_ratio=30.0, eig_its=20, pc="mg", mg_degree=2, stream=None):
+        """CG on the linear operator, preconditioned through the embedded-Q1 matrix: pc="mg" is a
+        multigrid V-cycle with Jacobi-Chebyshev smoothing, pc="cheb" Jacobi-Chebyshev alone."""
         o = SolveOpts(int(mixed), float(rtol), int(max_it), float(inner_rtol), int(max_outer), int(cheb_degree),
-                      float(cheb_ratio), int(eig_its))
+                      float(cheb_ratio), int(eig_its), {"cheb": 0, "mg": 1}[pc], int(mg_degree))
         st = SolveStats()
         check(_lib.dohpEllipSolve(self.h, q1.h, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(st), _stream(stream)))
         return {f: getattr(st, f) for f, _ in SolveStats._fields_}

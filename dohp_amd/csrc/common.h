@@ -53,6 +53,25 @@ template <typename T>
 struct DevBuf {
   T *ptr = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), n(o.n)
+  {
+    o.ptr = nullptr;
+    o.n = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept
+  {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      n = o.n;
+      o.ptr = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
   dErr alloc(size_t count) {
     release();
     n = count;
@@ -81,6 +100,24 @@ struct DevBuf {
 
 // Internal entry points shared between the operator and the Q1 assembly.
 extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hipStream_t st);
+
+// Sparse matrix on the owned nodes.  CSR is the assembled form; on a block space the owned nodes
+// form a box of the node lattice (nx * ny * nz, lexicographic, last index fastest) and every row
+// couples only its 27 lattice neighbours, so the matrix is also kept as a 27-point stencil:
+// st[s][r] with s = (di+1)*9 + (dj+1)*3 + (dk+1), zero where the neighbour is not owned.
+struct _p_dohpMat {
+  long nrows = 0, nnz = 0;
+  std::vector<int> h_rowptr;
+  dohp::DevBuf<int> rowptr, col;
+  dohp::DevBuf<double> val;
+  dohp::DevBuf<float> val32;
+  int box[3] = {0, 0, 0};
+  dohp::DevBuf<double> st64;
+  dohp::DevBuf<float> st32;
+};
+dErr dohpMatBuildStencil_Private(_p_dohpMat *A, dohpFS fs, hipStream_t st);
+dErr dohpStencilMult_Private(const _p_dohpMat *A, const double *x, double *y, hipStream_t st);
+dErr dohpStencilMultF32_Private(const _p_dohpMat *A, const float *x, float *y, hipStream_t st);
 
 // Function space on a structured hexahedral block.
 struct _p_dohpFS {

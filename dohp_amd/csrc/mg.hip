@@ -1,0 +1,591 @@
+// Geometric multigrid on the embedded-Q1 matrix of a block space: the multilevel solve that
+// ellip's driver obtains from PETSc's preconditioners applied to the Q1 matrix
+// (src/fs/tests/ellip.c:594-652,701-747).
+//
+// The owned nodes of a block space form a box of the GLL node lattice and the Q1 matrix is a
+// 27-point stencil on it.  Levels coarsen every box dimension d to floor(d/2): coarse node c
+// sits on fine node 2c+1; fine node 2c interpolates its neighbours c-1 and c with weight 1/2
+// (missing neighbours are Dirichlet zeros).  Coarse operators are Galerkin products P^T A P
+// (again 27-point), smoothing is a Jacobi-preconditioned Chebyshev polynomial on
+// [lo * lmax, hi * lmax] of D^{-1} A, the coarsest level (at most a few hundred nodes) is solved
+// with its explicit inverse.  Pre- and post-smoothers are the same polynomial, so the V-cycle is
+// a symmetric operator and serves as a CG preconditioner.
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+using namespace dohp;
+
+namespace {
+
+struct Box {
+  int nx, ny, nz;
+  __host__ __device__ long n() const { return (long)nx * ny * nz; }
+};
+
+__device__ __forceinline__ void unrank(const Box &b, long r, int &I, int &J, int &K)
+{
+  const long yz = (long)b.ny * b.nz;
+  I = (int)(r / yz);
+  const long rem = r - (long)I * yz;
+  J = (int)(rem / b.nz);
+  K = (int)(rem - (long)J * b.nz);
+}
+
+// CSR -> stencil (rows are box-ordered owned nodes, columns their owned neighbours).
+__global__ void csr_to_stencil_kernel(Box b, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                      const double *__restrict__ val, double *__restrict__ st,
+                                      float *__restrict__ st32)
+{
+  const long n = b.n();
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int s = 0; s < 27; s++) {
+    st[s * n + r] = 0.0;
+    st32[s * n + r] = 0.0f;
+  }
+  int I, J, K;
+  unrank(b, r, I, J, K);
+  for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
+    int I2, J2, K2;
+    unrank(b, col[k], I2, J2, K2);
+    const int di = I2 - I, dj = J2 - J, dk = K2 - K;
+    if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;  // not a lattice neighbour
+    const int s = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+    st[s * n + r] = val[k];
+    st32[s * n + r] = (float)val[k];
+  }
+}
+
+// y = A x.  Entries are summed in increasing column order, as the CSR product does.
+template <typename T>
+__global__ void stencil_mult_kernel(Box b, const T *__restrict__ A, const T *__restrict__ x, T *__restrict__ y)
+{
+  const long n = b.n();
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int I, J, K;
+  unrank(b, r, I, J, K);
+  const long sy = b.nz, sx = (long)b.ny * b.nz;
+  T acc = 0;
+#pragma unroll
+  for (int s = 0; s < 27; s++) {
+    const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
+    if (I + di < 0 || I + di >= b.nx || J + dj < 0 || J + dj >= b.ny || K + dk < 0 || K + dk >= b.nz) continue;
+    acc += A[s * n + r] * x[r + di * sx + dj * sy + dk];
+  }
+  y[r] = acc;
+}
+
+// r = b - A x  (b may be null: r = -A x)
+template <typename T>
+__global__ void stencil_residual_kernel(Box bx, const T *__restrict__ A, const T *__restrict__ b,
+                                        const T *__restrict__ x, T *__restrict__ r)
+{
+  const long n = bx.n();
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int I, J, K;
+  unrank(bx, i, I, J, K);
+  const long sy = bx.nz, sx = (long)bx.ny * bx.nz;
+  T acc = 0;
+#pragma unroll
+  for (int s = 0; s < 27; s++) {
+    const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
+    if (I + di < 0 || I + di >= bx.nx || J + dj < 0 || J + dj >= bx.ny || K + dk < 0 || K + dk >= bx.nz) continue;
+    acc += A[s * n + i] * x[i + di * sx + dj * sy + dk];
+  }
+  r[i] = (b ? b[i] : T(0)) - acc;
+}
+
+template <typename T>
+__global__ void diag_inv_kernel(long n, const T *__restrict__ A, T *__restrict__ dinv)
+{
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dinv[r] = T(1) / A[13 * n + r];
+}
+
+// Chebyshev step: d = a d + c dinv r;  x += d   (first step: a = 0)
+template <typename T>
+__global__ void cheb_step_kernel(long n, T a, T c, const T *__restrict__ dinv, const T *__restrict__ r,
+                                 T *__restrict__ d, T *__restrict__ x, int first_sets_x)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T dn = (a == T(0) ? T(0) : a * d[i]) + c * dinv[i] * r[i];
+  d[i] = dn;
+  x[i] = first_sets_x ? dn : x[i] + dn;
+}
+
+// 1-D prolongation weights of fine index f onto coarse indices (at most two).
+__device__ __forceinline__ int parents(int f, int nc, int (&c)[2], double (&w)[2])
+{
+  if (f & 1) {
+    c[0] = (f - 1) >> 1;
+    w[0] = 1.0;
+    return c[0] < nc ? 1 : 0;
+  }
+  int m = 0;
+  const int lo = (f >> 1) - 1, hi = f >> 1;
+  if (lo >= 0 && lo < nc) { c[m] = lo; w[m] = 0.5; m++; }
+  if (hi >= 0 && hi < nc) { c[m] = hi; w[m] = 0.5; m++; }
+  return m;
+}
+
+// bc = P^T rf
+template <typename T>
+__global__ void restrict_kernel(Box f, Box c, const T *__restrict__ rf, T *__restrict__ bc)
+{
+  const long C = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (C >= c.n()) return;
+  int I, J, K;
+  unrank(c, C, I, J, K);
+  const long sy = f.nz, sx = (long)f.ny * f.nz;
+  const int F0 = 2 * I + 1, F1 = 2 * J + 1, F2 = 2 * K + 1;
+  T acc = 0;
+  for (int a = -1; a <= 1; a++) {
+    if (F0 + a >= f.nx) continue;
+    for (int b = -1; b <= 1; b++) {
+      if (F1 + b >= f.ny) continue;
+      for (int d = -1; d <= 1; d++) {
+        if (F2 + d >= f.nz) continue;
+        const T w = T((a ? 0.5 : 1.0) * (b ? 0.5 : 1.0) * (d ? 0.5 : 1.0));
+        acc += w * rf[(F0 + a) * sx + (F1 + b) * sy + F2 + d];
+      }
+    }
+  }
+  bc[C] = acc;
+}
+
+// xf += P xc
+template <typename T>
+__global__ void prolong_add_kernel(Box f, Box c, const T *__restrict__ xc, T *__restrict__ xf)
+{
+  const long F = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (F >= f.n()) return;
+  int I, J, K;
+  unrank(f, F, I, J, K);
+  int ci[2], cj[2], ck[2];
+  double wi[2], wj[2], wk[2];
+  const int mi = parents(I, c.nx, ci, wi), mj = parents(J, c.ny, cj, wj), mk = parents(K, c.nz, ck, wk);
+  T acc = 0;
+  for (int a = 0; a < mi; a++)
+    for (int b = 0; b < mj; b++)
+      for (int d = 0; d < mk; d++)
+        acc += T(wi[a] * wj[b] * wk[d]) * xc[((long)ci[a] * c.ny + cj[b]) * c.nz + ck[d]];
+  xf[F] += acc;
+}
+
+// Ac = P^T Af P, one thread per coarse row (FP64).
+__global__ void rap_kernel(Box f, Box c, const double *__restrict__ Af, double *__restrict__ Ac)
+{
+  const long C = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nc = c.n(), nf = f.n();
+  if (C >= nc) return;
+  int I, J, K;
+  unrank(c, C, I, J, K);
+  double acc[27];
+  for (int s = 0; s < 27; s++) acc[s] = 0.0;
+  const long sy = f.nz, sx = (long)f.ny * f.nz;
+  for (int a = -1; a <= 1; a++) {
+    const int fi = 2 * I + 1 + a;
+    if (fi >= f.nx) continue;
+    for (int b = -1; b <= 1; b++) {
+      const int fj = 2 * J + 1 + b;
+      if (fj >= f.ny) continue;
+      for (int d = -1; d <= 1; d++) {
+        const int fk = 2 * K + 1 + d;
+        if (fk >= f.nz) continue;
+        const double wc = (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0) * (d ? 0.5 : 1.0);
+        const long fr = fi * sx + fj * sy + fk;
+        for (int s = 0; s < 27; s++) {
+          const int gi = fi + s / 9 - 1, gj = fj + (s / 3) % 3 - 1, gk = fk + s % 3 - 1;
+          if (gi < 0 || gi >= f.nx || gj < 0 || gj >= f.ny || gk < 0 || gk >= f.nz) continue;
+          const double afg = Af[s * nf + fr];
+          if (afg == 0.0) continue;
+          int pi[2], pj[2], pk[2];
+          double wi[2], wj[2], wk[2];
+          const int mi = parents(gi, c.nx, pi, wi), mj = parents(gj, c.ny, pj, wj), mk = parents(gk, c.nz, pk, wk);
+          for (int x = 0; x < mi; x++)
+            for (int y = 0; y < mj; y++)
+              for (int z = 0; z < mk; z++) {
+                const int ddi = pi[x] - I, ddj = pj[y] - J, ddk = pk[z] - K;  // in [-1, 1]
+                acc[(ddi + 1) * 9 + (ddj + 1) * 3 + (ddk + 1)] += wc * afg * wi[x] * wj[y] * wk[z];
+              }
+        }
+      }
+    }
+  }
+  for (int s = 0; s < 27; s++) Ac[s * nc + C] = acc[s];
+}
+
+template <typename T>
+__global__ void dense_mv_kernel(int n, const T *__restrict__ Ainv, const T *__restrict__ b, T *__restrict__ x)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  T acc = 0;
+  for (int c = 0; c < n; c++) acc += Ainv[(long)r * n + c] * b[c];
+  x[r] = acc;
+}
+
+template <typename T>
+__global__ void to_f32(long n, const double *__restrict__ a, float *__restrict__ b)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void sumsq_kernel(long n, const double *__restrict__ a, double *__restrict__ part)
+{
+  __shared__ double sh[256];
+  double s = 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) s += a[i] * a[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void scale_kernel(long n, double s, const double *__restrict__ dinv, const double *__restrict__ a,
+                             double *__restrict__ b)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = s * (dinv ? dinv[i] : 1.0) * a[i];
+}
+
+inline dim3 grid_of(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+
+dErr dohpMatBuildStencil_Private(_p_dohpMat *A, dohpFS fs, hipStream_t st)
+{
+  const long L0 = (long)fs->p * fs->M[0] + 1, L1 = (long)fs->p * fs->M[1] + 1, L2 = (long)fs->p * fs->M[2] + 1;
+  const int m = fs->dirichlet_mask;
+  Box b{(int)(L0 - ((m & 1) ? 2 : 0)), (int)(L1 - ((m & 1) ? 2 : 0)), (int)(L2 - ((m & 2) ? 1 : 0) - ((m & 4) ? 1 : 0))};
+  if (b.n() != A->nrows || b.nx <= 0 || b.ny <= 0 || b.nz <= 0) return 0;  // not a box: CSR only
+  A->box[0] = b.nx;
+  A->box[1] = b.ny;
+  A->box[2] = b.nz;
+  if (!A->st64.ptr) {
+    DOHP_CALL(A->st64.alloc(27 * b.n()));
+    DOHP_CALL(A->st32.alloc(27 * b.n()));
+  }
+  hipLaunchKernelGGL(csr_to_stencil_kernel, grid_of(b.n()), dim3(256), 0, st, b, A->rowptr.ptr, A->col.ptr,
+                     A->val.ptr, A->st64.ptr, A->st32.ptr);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+dErr dohpStencilMult_Private(const _p_dohpMat *A, const double *x, double *y, hipStream_t st)
+{
+  const Box b{A->box[0], A->box[1], A->box[2]};
+  hipLaunchKernelGGL(stencil_mult_kernel<double>, grid_of(b.n()), dim3(256), 0, st, b, A->st64.ptr, x, y);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+dErr dohpStencilMultF32_Private(const _p_dohpMat *A, const float *x, float *y, hipStream_t st)
+{
+  const Box b{A->box[0], A->box[1], A->box[2]};
+  hipLaunchKernelGGL(stencil_mult_kernel<float>, grid_of(b.n()), dim3(256), 0, st, b, A->st32.ptr, x, y);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+struct MGLevel {
+  Box box{};
+  long n = 0;
+  const double *A64 = nullptr;  // finest level: the Q1 matrix's own stencil
+  const float *A32 = nullptr;
+  DevBuf<double> own64, dinv64, b64, x64, r64, d64;
+  DevBuf<float> own32, dinv32, b32, x32, r32, d32;
+  double lmax = 0;
+};
+
+struct _p_dohpMG {
+  std::vector<MGLevel> L;
+  int degree = 2;
+  double lo = 0.1, hi = 1.1;
+  int ncoarse = 0;
+  DevBuf<double> cinv64;
+  DevBuf<float> cinv32;
+};
+
+namespace {
+
+dErr estimate_lmax(MGLevel &lv, hipStream_t st, int its)
+{
+  DevBuf<double> v, w, part;
+  DOHP_CALL(v.alloc(lv.n));
+  DOHP_CALL(w.alloc(lv.n));
+  DOHP_CALL(part.alloc(256));
+  std::vector<double> h(lv.n);
+  uint64_t s = 0x2545F4914F6CDD1DULL;
+  for (long i = 0; i < lv.n; i++) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    h[i] = (double)(s >> 11) / 9007199254740992.0 + 0.25;
+  }
+  DOHP_HIP(hipMemcpyAsync(v.ptr, h.data(), sizeof(double) * lv.n, hipMemcpyHostToDevice, st));
+  auto norm = [&](const double *a, double *out) -> dErr {
+    hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, st, lv.n, a, part.ptr);
+    double p[256];
+    DOHP_HIP(hipMemcpyAsync(p, part.ptr, sizeof(p), hipMemcpyDeviceToHost, st));
+    DOHP_HIP(hipStreamSynchronize(st));
+    double t = 0;
+    for (int i = 0; i < 256; i++) t += p[i];
+    *out = std::sqrt(t);
+    return 0;
+  };
+  double nv, nw = 0;
+  DOHP_CALL(norm(v.ptr, &nv));
+  hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0 / nv, nullptr, v.ptr, v.ptr);
+  for (int k = 0; k < its; k++) {
+    hipLaunchKernelGGL(stencil_mult_kernel<double>, grid_of(lv.n), dim3(256), 0, st, lv.box, lv.A64, v.ptr, w.ptr);
+    hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0, lv.dinv64.ptr, w.ptr, w.ptr);
+    DOHP_CALL(norm(w.ptr, &nw));
+    hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0 / nw, nullptr, w.ptr, v.ptr);
+  }
+  DOHP_HIP(hipGetLastError());
+  lv.lmax = nw;
+  return 0;
+}
+
+// Explicit inverse of the (symmetric positive definite) coarsest matrix by Gauss-Jordan, host FP64.
+dErr coarse_inverse(const MGLevel &lv, hipStream_t st, std::vector<double> &inv)
+{
+  const long n = lv.n;
+  DOHP_CHECK(n <= 4096, dohpERR_ARG, "multigrid: the coarsest level is too large for a direct solve");
+  std::vector<double> S(27 * n);
+  DOHP_HIP(hipMemcpyAsync(S.data(), lv.A64, sizeof(double) * 27 * n, hipMemcpyDeviceToHost, st));
+  DOHP_HIP(hipStreamSynchronize(st));
+  std::vector<double> A(n * n, 0.0);
+  const Box b = lv.box;
+  for (long r = 0; r < n; r++) {
+    const int I = (int)(r / ((long)b.ny * b.nz)), J = (int)((r / b.nz) % b.ny), K = (int)(r % b.nz);
+    for (int s = 0; s < 27; s++) {
+      const int i2 = I + s / 9 - 1, j2 = J + (s / 3) % 3 - 1, k2 = K + s % 3 - 1;
+      if (i2 < 0 || i2 >= b.nx || j2 < 0 || j2 >= b.ny || k2 < 0 || k2 >= b.nz) continue;
+      A[r * n + ((long)i2 * b.ny + j2) * b.nz + k2] += S[s * n + r];
+    }
+  }
+  inv.assign(n * n, 0.0);
+  for (long i = 0; i < n; i++) inv[i * n + i] = 1.0;
+  for (long k = 0; k < n; k++) {
+    const double piv = A[k * n + k];
+    DOHP_CHECK(piv > 0.0, dohpERR_GENERIC, "multigrid: coarsest matrix is not positive definite");
+    const double ip = 1.0 / piv;
+    for (long j = 0; j < n; j++) {
+      A[k * n + j] *= ip;
+      inv[k * n + j] *= ip;
+    }
+    for (long i = 0; i < n; i++) {
+      if (i == k) continue;
+      const double f = A[i * n + k];
+      if (f == 0.0) continue;
+      for (long j = 0; j < n; j++) {
+        A[i * n + j] -= f * A[k * n + j];
+        inv[i * n + j] -= f * inv[k * n + j];
+      }
+    }
+  }
+  return 0;
+}
+
+template <typename T>
+struct LvPtr;
+template <>
+struct LvPtr<double> {
+  static const double *A(const MGLevel &l) { return l.A64; }
+  static const double *dinv(const MGLevel &l) { return l.dinv64.ptr; }
+  static double *b(MGLevel &l) { return l.b64.ptr; }
+  static double *x(MGLevel &l) { return l.x64.ptr; }
+  static double *r(MGLevel &l) { return l.r64.ptr; }
+  static double *d(MGLevel &l) { return l.d64.ptr; }
+  static const double *cinv(const _p_dohpMG *m) { return m->cinv64.ptr; }
+};
+template <>
+struct LvPtr<float> {
+  static const float *A(const MGLevel &l) { return l.A32; }
+  static const float *dinv(const MGLevel &l) { return l.dinv32.ptr; }
+  static float *b(MGLevel &l) { return l.b32.ptr; }
+  static float *x(MGLevel &l) { return l.x32.ptr; }
+  static float *r(MGLevel &l) { return l.r32.ptr; }
+  static float *d(MGLevel &l) { return l.d32.ptr; }
+  static const float *cinv(const _p_dohpMG *m) { return m->cinv32.ptr; }
+};
+
+// x <- x + p(D^-1 A) D^-1 (b - A x), Chebyshev of the given degree on [lo, hi] * lmax.
+template <typename T>
+void smooth(const _p_dohpMG *mg, MGLevel &lv, const T *b, T *x, bool zero_init, hipStream_t st)
+{
+  using P = LvPtr<T>;
+  const double lmin = mg->lo * lv.lmax, lmaxs = mg->hi * lv.lmax;
+  const double theta = 0.5 * (lmaxs + lmin), delta = 0.5 * (lmaxs - lmin), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  T *r = P::r(lv), *d = P::d(lv);
+  const dim3 g = grid_of(lv.n);
+  if (zero_init)
+    hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, T(0), (T)(1.0 / theta), P::dinv(lv), b, d, x, 1);
+  else {
+    hipLaunchKernelGGL(stencil_residual_kernel<T>, g, dim3(256), 0, st, lv.box, P::A(lv), b, x, r);
+    hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, T(0), (T)(1.0 / theta), P::dinv(lv), r, d, x, 0);
+  }
+  for (int k = 1; k < mg->degree; k++) {
+    hipLaunchKernelGGL(stencil_residual_kernel<T>, g, dim3(256), 0, st, lv.box, P::A(lv), b, x, r);
+    const double rho_new = 1.0 / (2.0 * sigma - rho);
+    hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, (T)(rho_new * rho), (T)(2.0 * rho_new / delta),
+                       P::dinv(lv), r, d, x, 0);
+    rho = rho_new;
+  }
+}
+
+template <typename T>
+void vcycle(_p_dohpMG *mg, size_t l, const T *b, T *x, hipStream_t st)
+{
+  using P = LvPtr<T>;
+  MGLevel &lv = mg->L[l];
+  if (l + 1 == mg->L.size()) {
+    hipLaunchKernelGGL(dense_mv_kernel<T>, dim3((unsigned)((lv.n + 127) / 128)), dim3(128), 0, st, (int)lv.n,
+                       P::cinv(mg), b, x);
+    return;
+  }
+  MGLevel &cv = mg->L[l + 1];
+  smooth<T>(mg, lv, b, x, true, st);
+  hipLaunchKernelGGL(stencil_residual_kernel<T>, grid_of(lv.n), dim3(256), 0, st, lv.box, P::A(lv), b, x, P::r(lv));
+  hipLaunchKernelGGL(restrict_kernel<T>, grid_of(cv.n), dim3(256), 0, st, lv.box, cv.box, P::r(lv), P::b(cv));
+  vcycle<T>(mg, l + 1, P::b(cv), P::x(cv), st);
+  hipLaunchKernelGGL(prolong_add_kernel<T>, grid_of(lv.n), dim3(256), 0, st, lv.box, cv.box, P::x(cv), x);
+  smooth<T>(mg, lv, b, x, false, st);
+}
+
+}  // namespace
+
+extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohpMG *out, void *stream)
+{
+  DOHP_CHECK(q1 && out, dohpERR_ARG, "null argument");
+  DOHP_CHECK(q1->st64.ptr, dohpERR_ARG, "multigrid needs the stencil form of a block-space Q1 matrix");
+  hipStream_t st = (hipStream_t)stream;
+  *out = nullptr;
+  _p_dohpMG *mg = new _p_dohpMG;
+  mg->degree = smooth_degree > 0 ? smooth_degree : 2;
+  const long nmax = max_coarse > 0 ? max_coarse : 512;
+  dErr err = 0;
+  {
+    MGLevel lv;
+    lv.box = Box{q1->box[0], q1->box[1], q1->box[2]};
+    lv.n = lv.box.n();
+    lv.A64 = q1->st64.ptr;
+    lv.A32 = q1->st32.ptr;
+    mg->L.push_back(std::move(lv));
+  }
+  while (!err) {
+    MGLevel &f = mg->L.back();
+    if (f.n <= nmax) break;
+    const Box cb{f.box.nx / 2, f.box.ny / 2, f.box.nz / 2};
+    if (cb.nx < 1 || cb.ny < 1 || cb.nz < 1) break;
+    MGLevel c;
+    c.box = cb;
+    c.n = cb.n();
+    err = c.own64.alloc(27 * c.n);
+    if (!err) err = c.own32.alloc(27 * c.n);
+    if (err) break;
+    hipLaunchKernelGGL(rap_kernel, grid_of(c.n), dim3(256), 0, st, f.box, cb, f.A64, c.own64.ptr);
+    hipLaunchKernelGGL(to_f32<float>, grid_of(27 * c.n), dim3(256), 0, st, 27 * c.n, c.own64.ptr, c.own32.ptr);
+    c.A64 = c.own64.ptr;
+    c.A32 = c.own32.ptr;
+    mg->L.push_back(std::move(c));
+  }
+  for (size_t l = 0; !err && l < mg->L.size(); l++) {
+    MGLevel &lv = mg->L[l];
+    const long n = lv.n;
+    err = lv.dinv64.alloc(n);
+    if (!err) err = lv.dinv32.alloc(n);
+    if (!err) err = lv.r64.alloc(n);
+    if (!err) err = lv.d64.alloc(n);
+    if (!err) err = lv.r32.alloc(n);
+    if (!err) err = lv.d32.alloc(n);
+    if (!err && l > 0) err = lv.b64.alloc(n);
+    if (!err && l > 0) err = lv.x64.alloc(n);
+    if (!err && l > 0) err = lv.b32.alloc(n);
+    if (!err && l > 0) err = lv.x32.alloc(n);
+    if (err) break;
+    hipLaunchKernelGGL(diag_inv_kernel<double>, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.dinv64.ptr);
+    hipLaunchKernelGGL(to_f32<float>, grid_of(n), dim3(256), 0, st, n, lv.dinv64.ptr, lv.dinv32.ptr);
+    if (l + 1 < mg->L.size()) err = estimate_lmax(lv, st, 15);
+  }
+  if (!err) {
+    std::vector<double> inv;
+    err = coarse_inverse(mg->L.back(), st, inv);
+    mg->ncoarse = (int)mg->L.back().n;
+    if (!err) err = mg->cinv64.upload(inv.data(), inv.size());
+    std::vector<float> inv32(inv.begin(), inv.end());
+    if (!err) err = mg->cinv32.upload(inv32.data(), inv32.size());
+  }
+  if (!err && hipStreamSynchronize(st) != hipSuccess) {
+    set_error("multigrid setup failed");
+    err = dohpERR_HIP;
+  }
+  if (err) {
+    delete mg;
+    return err;
+  }
+  *out = mg;
+  return 0;
+}
+
+extern "C" dErr dohpMGGetInfo(dohpMG mg, int *nlevels, long *ncoarse, double *lmax_fine)
+{
+  DOHP_CHECK(mg, dohpERR_ARG, "null multigrid");
+  if (nlevels) *nlevels = (int)mg->L.size();
+  if (ncoarse) *ncoarse = mg->ncoarse;
+  if (lmax_fine) *lmax_fine = mg->L[0].lmax;
+  return 0;
+}
+
+extern "C" dErr dohpMGGetLevel(dohpMG mg, int level, int *box, double *stencil_host)
+{
+  DOHP_CHECK(mg && level >= 0 && level < (int)mg->L.size(), dohpERR_ARG, "no such multigrid level");
+  const MGLevel &lv = mg->L[level];
+  if (box) {
+    box[0] = lv.box.nx;
+    box[1] = lv.box.ny;
+    box[2] = lv.box.nz;
+  }
+  if (stencil_host) {
+    DOHP_HIP(hipDeviceSynchronize());
+    DOHP_HIP(hipMemcpy(stencil_host, lv.A64, sizeof(double) * 27 * lv.n, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+extern "C" dErr dohpMGApply(dohpMG mg, const double *r, double *z, void *stream)
+{
+  DOHP_CHECK(mg && r && z, dohpERR_ARG, "null argument");
+  vcycle<double>(mg, 0, r, z, (hipStream_t)stream);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" dErr dohpMGApplyF32(dohpMG mg, const float *r, float *z, void *stream)
+{
+  DOHP_CHECK(mg && r && z, dohpERR_ARG, "null argument");
+  vcycle<float>(mg, 0, r, z, (hipStream_t)stream);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" dErr dohpMGDestroy(dohpMG *mg)
+{
+  if (mg && *mg) {
+    delete *mg;
+    *mg = nullptr;
+  }
+  return 0;
+}

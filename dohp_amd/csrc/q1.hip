@@ -19,13 +19,6 @@
 
 using namespace dohp;
 
-struct _p_dohpMat {
-  long nrows = 0, nnz = 0;
-  std::vector<int> h_rowptr;
-  DevBuf<int> rowptr, col;
-  DevBuf<double> val;
-  DevBuf<float> val32;
-};
 
 namespace {
 
@@ -360,7 +353,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
                      A->val.ptr, tb, ph);
   hipLaunchKernelGGL(to_f32_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->val32.ptr, A->nnz);
   DOHP_HIP(hipGetLastError());
-  return 0;
+  return dohpMatBuildStencil_Private(A, fs, st);
 }
 
 extern "C" dErr dohpMatGetSizes(dohpMat A, long *nrows, long *nnz)
@@ -384,6 +377,7 @@ extern "C" dErr dohpMatGetCSR(dohpMat A, int *rowptr, int *col, double *val)
 extern "C" dErr dohpMatMult(dohpMat A, const double *x, double *y, void *stream)
 {
   DOHP_CHECK(A && x && y, dohpERR_ARG, "null argument");
+  if (A->st64.ptr) return dohpStencilMult_Private(A, x, y, (hipStream_t)stream);
   hipLaunchKernelGGL(csr_spmv_kernel<double>, dim3((A->nrows + 255) / 256), dim3(256), 0, (hipStream_t)stream, A->nrows,
                      A->rowptr.ptr, A->col.ptr, A->val.ptr, x, y);
   DOHP_HIP(hipGetLastError());
@@ -393,6 +387,7 @@ extern "C" dErr dohpMatMult(dohpMat A, const double *x, double *y, void *stream)
 extern "C" dErr dohpMatMultF32(dohpMat A, const float *x, float *y, void *stream)
 {
   DOHP_CHECK(A && x && y, dohpERR_ARG, "null argument");
+  if (A->st32.ptr) return dohpStencilMultF32_Private(A, x, y, (hipStream_t)stream);
   hipLaunchKernelGGL(csr_spmv_kernel<float>, dim3((A->nrows + 255) / 256), dim3(256), 0, (hipStream_t)stream, A->nrows,
                      A->rowptr.ptr, A->col.ptr, A->val32.ptr, x, y);
   DOHP_HIP(hipGetLastError());

@@ -1,9 +1,10 @@
 // Krylov drivers above the operator: the role PETSc's KSP plays for ellip
 // (src/fs/tests/ellip.c:701-747), on device vectors.
 //
-//   * PCG with the embedded-Q1 matrix as preconditioner, applied approximately by a
+//   * PCG with the embedded-Q1 matrix as preconditioner, applied approximately either by a
 //     Jacobi-preconditioned Chebyshev iteration of fixed degree (a fixed polynomial in
-//     D^{-1} A_q1 times D^{-1}, hence symmetric, so CG stays valid).
+//     D^{-1} A_q1 times D^{-1}, hence symmetric, so CG stays valid) or by a multigrid V-cycle
+//     on the Q1 matrix with Jacobi-Chebyshev smoothing (symmetric as well).
 //   * Plain path: everything in FP64.
 //   * Mixed path: FP64 outer defect correction x += A32^{-1}(b - A64 x) with the inner PCG in
 //     FP32 (FP32 operator and Q1 applies); residuals of the outer loop and every Krylov dot
@@ -163,6 +164,24 @@ struct Cheb {
   }
 };
 
+// Preconditioner of one precision: Jacobi-Chebyshev on the Q1 matrix, or a multigrid V-cycle on it.
+template <typename T>
+struct Prec {
+  Cheb<T> *cheb = nullptr;
+  dohpMG mg = nullptr;
+  dErr apply(Ctx &c, const T *r, T *z);
+};
+template <>
+dErr Prec<double>::apply(Ctx &c, const double *r, double *z)
+{
+  return mg ? dohpMGApply(mg, r, z, c.st) : cheb->apply(c, r, z);
+}
+template <>
+dErr Prec<float>::apply(Ctx &c, const float *r, float *z)
+{
+  return mg ? dohpMGApplyF32(mg, r, z, c.st) : cheb->apply(c, r, z);
+}
+
 // Largest eigenvalue of D^{-1} A_q1 by power iteration (FP64).
 dErr estimate_lmax(Ctx &c, dohpMat q1, const double *dinv, int its, double *lmax)
 {
@@ -194,7 +213,7 @@ dErr estimate_lmax(Ctx &c, dohpMat q1, const double *dinv, int its, double *lmax
 
 // PCG on A x = b (x holds the initial guess).  Stops at ||r|| <= rtol ||b|| or max_it.
 template <typename T>
-dErr pcg(Ctx &c, dohpEllip op, Cheb<T> &M, const T *b, T *x, double rtol, int max_it, int *its, double *rnorm,
+dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, const T *b, T *x, double rtol, int max_it, int *its, double *rnorm,
          double *bnorm)
 {
   DevBuf<T> r, z, p, q;
@@ -267,13 +286,21 @@ extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double
   const double lmin = lmax / (opts->cheb_ratio > 1 ? opts->cheb_ratio : 30.0);
   stats->lmax = lmax;
   const int degree = opts->cheb_degree > 0 ? opts->cheb_degree : 4;
+  struct MGHolder {
+    dohpMG mg = nullptr;
+    ~MGHolder() { dohpMGDestroy(&mg); }
+  } mgh;
+  if (opts->pc == 1) DOHP_CALL(dohpMGCreate(q1, opts->mg_degree, 0, &mgh.mg, c.st));
   const auto t_setup = std::chrono::steady_clock::now();
   stats->setup_seconds = std::chrono::duration<double>(t_setup - t0).count();
   if (!opts->mixed) {
-    Cheb<double> M{q1, dinv.ptr, lmin, lmax, degree};
-    DOHP_CALL(M.res.alloc(c.n));
-    DOHP_CALL(M.d.alloc(c.n));
-    DOHP_CALL(M.w.alloc(c.n));
+    Cheb<double> C64{q1, dinv.ptr, lmin, lmax, degree};
+    DOHP_CALL(C64.res.alloc(c.n));
+    DOHP_CALL(C64.d.alloc(c.n));
+    DOHP_CALL(C64.w.alloc(c.n));
+    Prec<double> M;
+    M.cheb = &C64;
+    M.mg = mgh.mg;
     int its;
     double rn, bn;
     DOHP_CALL(pcg<double>(c, op, M, b, x, opts->rtol, opts->max_it, &its, &rn, &bn));
@@ -282,10 +309,13 @@ extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double
     stats->rnorm = rn;
     stats->bnorm = bn;
   } else {
-    Cheb<float> M{q1, dinv32.ptr, lmin, lmax, degree};
-    DOHP_CALL(M.res.alloc(c.n));
-    DOHP_CALL(M.d.alloc(c.n));
-    DOHP_CALL(M.w.alloc(c.n));
+    Cheb<float> C32{q1, dinv32.ptr, lmin, lmax, degree};
+    DOHP_CALL(C32.res.alloc(c.n));
+    DOHP_CALL(C32.d.alloc(c.n));
+    DOHP_CALL(C32.w.alloc(c.n));
+    Prec<float> M;
+    M.cheb = &C32;
+    M.mg = mgh.mg;
     DevBuf<double> r, t;
     DevBuf<float> r32, e32;
     DOHP_CALL(r.alloc(c.n));

@@ -134,6 +134,23 @@ dErr dohpMatMultF32(dohpMat A, const float *x_dev, float *y_dev, void *stream);
 dErr dohpMatGetDiagonal(dohpMat A, double *d_dev, void *stream);
 dErr dohpMatDestroy(dohpMat *A);
 
+/* ------------------------------------------------------------ Q1 multigrid */
+typedef struct _p_dohpMG *dohpMG;
+
+/* Geometric multigrid on the embedded-Q1 matrix of a block space (the multilevel preconditioner
+ * ellip's driver takes from PETSc for its Q1 matrix, src/fs/tests/ellip.c:594-652,701-747).
+ * Levels halve the owned-node box, Galerkin coarse operators P^T A P, Jacobi-Chebyshev smoothing
+ * of the given degree on [0.1, 1.1] * lmax(D^-1 A), explicit inverse on a coarsest level of at
+ * most max_coarse nodes (0 = 512).  The V-cycle is symmetric (usable inside CG).  One apply at a
+ * time per object: the levels' work vectors are shared. */
+dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohpMG *mg, void *stream);
+dErr dohpMGGetInfo(dohpMG mg, int *nlevels, long *ncoarse, double *lmax_fine);
+/* Box dimensions and 27-point stencil [27][n] (FP64) of level `level` (0 = the Q1 matrix). */
+dErr dohpMGGetLevel(dohpMG mg, int level, int *box, double *stencil_host);
+dErr dohpMGApply(dohpMG mg, const double *r_dev, double *z_dev, void *stream);
+dErr dohpMGApplyF32(dohpMG mg, const float *r_dev, float *z_dev, void *stream);
+dErr dohpMGDestroy(dohpMG *mg);
+
 /* ------------------------------------------------------------------ solvers */
 typedef struct {
   int mixed;          /* 0: plain FP64 PCG; 1: FP32 inner PCG inside FP64 defect correction */
@@ -144,6 +161,8 @@ typedef struct {
   int cheb_degree;    /* degree of the Jacobi-Chebyshev iteration on the Q1 matrix */
   double cheb_ratio;  /* Chebyshev interval [lmax/ratio, lmax] of D^{-1} A_q1 */
   int eig_its;        /* power iterations for lmax */
+  int pc;             /* 0: Jacobi-Chebyshev on the Q1 matrix; 1: multigrid V-cycle on it (dohpMG) */
+  int mg_degree;      /* Chebyshev degree of the multigrid smoother (pre and post) */
 } dohpSolveOpts;
 
 typedef struct {

@@ -1,6 +1,9 @@
-"""Krylov solve with the embedded-Q1 preconditioner: plain FP64 and FP32-inner/FP64-outer."""
+"""Krylov solve with the embedded-Q1 preconditioner (Jacobi-Chebyshev, or a multigrid V-cycle on
+the Q1 matrix): plain FP64 and FP32-inner/FP64-outer; the multigrid's stencil form, Galerkin coarse
+operators and symmetry."""
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -16,27 +19,95 @@ def C():
     return capi
 
 
-@pytest.mark.parametrize("M,deg", [((6, 6, 6), 5), ((4, 5, 3), 3)])
-def test_solve_plain_and_mixed_reach_1e10(C, M, deg):
-    print(f"solve M={M} p={deg}", flush=True)
+def _problem(C, M, deg, seed=7):
     V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=3)
     fs = C.FS(M, deg, V)
-    k = lognormal_field(V, seed=7)
-    op = C.Ellip(fs, p=2.0, kvert=k)
+    op = C.Ellip(fs, p=2.0, kvert=lognormal_field(V, seed=seed))
     xs = torch.as_tensor(2 * splitmix64_uniform(5, fs.nowned) - 1, dtype=torch.float64, device="cuda")
     b = torch.zeros_like(xs)
     op.mat_mult(xs, b)
     q1 = op.assemble_q1(torch.zeros_like(xs))
-    res = {}
-    for mixed in (False, True):
-        x = torch.zeros_like(xs)
-        st = C.Ellip.solve(op, q1, b, x, mixed=mixed, rtol=1e-10)
-        r = torch.zeros_like(xs)
-        op.mat_mult(x, r)
-        relres = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
-        err = float(torch.linalg.norm(x - xs) / torch.linalg.norm(xs))
-        print(f"mixed={mixed} stats={st} relres={relres:.2e} err={err:.2e}")
-        assert relres <= 1.01e-10
-        assert err < 1e-6
-        res[mixed] = st
-    assert res[False]["inner_its"] < 1000
+    return fs, op, q1, xs, b
+
+
+def _interp_1d(nf):
+    nc = nf // 2
+    P = np.zeros((nf, nc))
+    for f in range(nf):
+        if f % 2:
+            if (f - 1) // 2 < nc:
+                P[f, (f - 1) // 2] = 1.0
+        else:
+            for c in (f // 2 - 1, f // 2):
+                if 0 <= c < nc:
+                    P[f, c] = 0.5
+    return P
+
+
+def _stencil_to_csr(box, st):
+    nx, ny, nz = box
+    n = nx * ny * nz
+    I, J, K = np.unravel_index(np.arange(n), box)
+    rows, cols, vals = [], [], []
+    for s in range(27):
+        di, dj, dk = s // 9 - 1, (s // 3) % 3 - 1, s % 3 - 1
+        ok = (I + di >= 0) & (I + di < nx) & (J + dj >= 0) & (J + dj < ny) & (K + dk >= 0) & (K + dk < nz)
+        r = np.nonzero(ok)[0]
+        rows.append(r)
+        cols.append(r + (di * ny + dj) * nz + dk)
+        vals.append(st[s, r])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
+def test_stencil_product_equals_csr(C):
+    fs, op, q1, xs, b = _problem(C, (3, 4, 3), 3)
+    A = q1.to_scipy()
+    y = torch.zeros_like(xs)
+    q1.mult(xs, y)
+    ref = A @ xs.cpu().numpy()
+    np.testing.assert_allclose(y.cpu().numpy(), ref, rtol=0, atol=1e-14 * np.abs(ref).max())
+
+
+def test_multigrid_galerkin_levels_and_symmetry(C):
+    fs, op, q1, xs, b = _problem(C, (4, 4, 4), 4)
+    mg = C.MG(q1, smooth_degree=2, max_coarse=64)
+    assert mg.nlevels >= 3
+    box0, st0 = mg.level(0)
+    A = _stencil_to_csr(box0, st0)
+    np.testing.assert_allclose((A - q1.to_scipy()).toarray(), 0, atol=1e-13 * abs(A).max())
+    for l in range(1, mg.nlevels):
+        box1, st1 = mg.level(l)
+        P = sp.csr_matrix(np.kron(np.kron(_interp_1d(box0[0]), _interp_1d(box0[1])), _interp_1d(box0[2])))
+        Ac = (P.T @ A @ P).toarray()
+        got = _stencil_to_csr(box1, st1).toarray()
+        np.testing.assert_allclose(got, Ac, atol=1e-12 * np.abs(Ac).max())
+        A, box0 = sp.csr_matrix(Ac), box1
+    # the V-cycle is a symmetric operator: <M r1, r2> = <r1, M r2>
+    r1 = torch.as_tensor(splitmix64_uniform(1, fs.nowned) - 0.5, dtype=torch.float64, device="cuda")
+    r2 = torch.as_tensor(splitmix64_uniform(2, fs.nowned) - 0.5, dtype=torch.float64, device="cuda")
+    z1, z2 = torch.zeros_like(r1), torch.zeros_like(r2)
+    mg.apply(r1, z1)
+    mg.apply(r2, z2)
+    a, c = float(torch.dot(z1, r2)), float(torch.dot(r1, z2))
+    assert abs(a - c) <= 1e-11 * abs(a)
+    assert float(torch.dot(z1, r1)) > 0
+
+
+@pytest.mark.parametrize("M,deg", [((6, 6, 6), 5), ((4, 5, 3), 3)])
+def test_solve_plain_and_mixed_reach_1e10(C, M, deg):
+    print(f"solve M={M} p={deg}", flush=True)
+    fs, op, q1, xs, b = _problem(C, M, deg)
+    its = {}
+    for pc in ("cheb", "mg"):
+        for mixed in (False, True):
+            x = torch.zeros_like(xs)
+            st = op.solve(q1, b, x, mixed=mixed, rtol=1e-10, pc=pc)
+            r = torch.zeros_like(xs)
+            op.mat_mult(x, r)
+            relres = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
+            err = float(torch.linalg.norm(x - xs) / torch.linalg.norm(xs))
+            print(f"pc={pc} mixed={mixed} stats={st} relres={relres:.2e} err={err:.2e}")
+            assert relres <= 1.01e-10
+            assert err < 1e-6
+            its[(pc, mixed)] = st["inner_its"]
+    assert its[("mg", False)] < its[("cheb", False)]

@@ -73,6 +73,8 @@ _proto("dohpMatMult", _vp, _vp, _vp, _vp)
 _proto("dohpMatMultF32", _vp, _vp, _vp, _vp)
 _proto("dohpMatGetDiagonal", _vp, _vp, _vp)
 _proto("dohpMatDestroy", ctypes.POINTER(_vp))
+
+
 class MG:
     """Multigrid V-cycle on the embedded-Q1 matrix (dohpMG)."""
 
@@ -106,13 +108,13 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("mixed", ctypes.c_int), ("rtol", ctypes.c_double), ("max_it", ctypes.c_int),
                 ("inner_rtol", ctypes.c_double), ("max_outer", ctypes.c_int), ("cheb_degree", ctypes.c_int),
                 ("cheb_ratio", ctypes.c_double), ("eig_its", ctypes.c_int), ("pc", ctypes.c_int),
-                ("mg_degree", ctypes.c_int)]
+                ("mg_degree", ctypes.c_int), ("fallback_its", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("outer_its", ctypes.c_int), ("inner_its", ctypes.c_int), ("bnorm", ctypes.c_double),
                 ("rnorm", ctypes.c_double), ("lmax", ctypes.c_double), ("setup_seconds", ctypes.c_double),
-                ("seconds", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("fallback", ctypes.c_int)]
 
 
 _proto("dohpMGCreate", _vp, ctypes.c_int, ctypes.c_int, _vp, _vp)
@@ -122,8 +124,91 @@ _proto("dohpMGApply", _vp, _vp, _vp, _vp)
 _proto("dohpMGApplyF32", _vp, _vp, _vp, _vp)
 _proto("dohpMGDestroy", _vp)
 _proto("dohpEllipSolve", _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)
-_lib.dohpGetLastError.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
-_lib.dohpGetLastError.restype = None
+_proto("dohpEllipSolveLearned", _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)
+_proto("dohpEllipSolveRecord", _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp,
+       ctypes.c_int, ctypes.c_int, _vp, _vp)
+_lib.dohpMLPNumParams.argtypes = []
+_lib.dohpMLPNumParams.restype = ctypes.c_int
+_lib.dohpMLPNumInputs.argtypes = []
+_lib.dohpMLPNumInputs.restype = ctypes.c_int
+_proto("dohpMLPCreate", _vp, _vp)
+_proto("dohpMLPGetParams", _vp, _vp, _vp)
+_proto("dohpMLPDestroy", _vp)
+_proto("dohpMLPNumPatches", _vp, _vp)
+_proto("dohpMLPFeatures", _vp, _vp, _vp, _vp, _vp)
+_proto("dohpMLPWeights", _vp, _vp, _vp, _vp, _vp, _vp)
+_proto("dohpMLPGradient", _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp)
+_proto("dohpMLPEnergyWeightGradient", _vp, _vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp)
+_proto("dohpMLPWeightStep", _vp, _vp, _vp, _vp, ctypes.c_float, _vp)
+_proto("dohpMLPEnergyStep", _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_float, _vp, _vp)
+
+
+class MLP:
+    """Learned node weights of the Q1 multigrid preconditioner, z = S M(S r) (dohpMLP)."""
+
+    nparam = _lib.dohpMLPNumParams()
+    ninputs = _lib.dohpMLPNumInputs()
+
+    def __init__(self, params=None):
+        self.h = ctypes.c_void_p()
+        p = None
+        if params is not None:
+            self._p = np.ascontiguousarray(params, dtype=np.float32)
+            assert self._p.size == self.nparam
+            p = self._p.ctypes.data_as(ctypes.c_void_p)
+        check(_lib.dohpMLPCreate(p, ctypes.byref(self.h)))
+
+    def params(self):
+        m, w = np.zeros(self.nparam, np.float32), np.zeros(self.nparam, np.float32)
+        check(_lib.dohpMLPGetParams(self.h, m.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(ctypes.c_void_p)))
+        return m, w
+
+    @staticmethod
+    def num_patches(q1):
+        n = ctypes.c_long()
+        check(_lib.dohpMLPNumPatches(q1.h, ctypes.byref(n)))
+        return n.value
+
+    @staticmethod
+    def features(q1, fs, dinv, X, stream=None):
+        check(_lib.dohpMLPFeatures(q1.h, fs.h, _ptr(dinv), _ptr(X), _stream(stream)))
+
+    def weights(self, q1, X, s, s32=None, stream=None):
+        check(_lib.dohpMLPWeights(self.h, q1.h, _ptr(X), _ptr(s), _ptr(s32) if s32 is not None else None,
+                                  _stream(stream)))
+
+    def gradient(self, X, T, B=None, stream=None):
+        """MSE gradient of the raw outputs (checks the training kernels)."""
+        B = int(B or X.shape[0])
+        g = np.zeros(self.nparam, np.float32)
+        loss = ctypes.c_double()
+        check(_lib.dohpMLPGradient(self.h, B, _ptr(X), _ptr(T), g.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss),
+                                   _stream(stream)))
+        return loss.value, g
+
+    @staticmethod
+    def energy_weight_gradient(op, mg, s, r, e, alpha=0.0, gs=None, stream=None):
+        """(E, (z.r, z.Jz, e.Je)) for z = S M(S r); gs receives dE/ds if given."""
+        E = ctypes.c_double()
+        terms = (ctypes.c_double * 3)()
+        check(_lib.dohpMLPEnergyWeightGradient(op.h, mg.h, int(s.shape[0]), _ptr(s), _ptr(r), _ptr(e),
+                                               ctypes.c_double(alpha), _ptr(gs) if gs is not None else None,
+                                               ctypes.byref(E), terms, _stream(stream)))
+        return E.value, tuple(terms)
+
+    def weight_step(self, q1, X, gs, lr, stream=None):
+        """Adam step from dL/ds at the node weights."""
+        check(_lib.dohpMLPWeightStep(self.h, q1.h, _ptr(X), _ptr(gs), ctypes.c_float(lr), _stream(stream)))
+
+    def energy_step(self, op, q1, mg, X, r, e, lr, alpha=0.0, stream=None):
+        E = ctypes.c_double()
+        check(_lib.dohpMLPEnergyStep(self.h, op.h, q1.h, mg.h, _ptr(X), _ptr(r), _ptr(e), ctypes.c_double(alpha),
+                                     ctypes.c_float(lr), ctypes.byref(E), _stream(stream)))
+        return E.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.dohpMLPDestroy(ctypes.byref(self.h))
 
 
 class DohpError(RuntimeError):
@@ -315,14 +400,31 @@ class Ellip:
         return m
 
     def solve(self, q1, b, x, mixed=False, rtol=1e-10, max_it=2000, inner_rtol=1e-4, max_outer=50, cheb_degree=4,
-              cheb_ratio=30.0, eig_its=20, pc="mg", mg_degree=2, stream=None):
+              cheb_ratio=30.0, eig_its=20, pc="mg", mg_degree=2, mlp=None, fallback_its=0, record=None,
+              stream=None):
         """CG on the linear operator, preconditioned through the embedded-Q1 matrix: pc="mg" is a
-        multigrid V-cycle with Jacobi-Chebyshev smoothing, pc="cheb" Jacobi-Chebyshev alone."""
+        multigrid V-cycle with Jacobi-Chebyshev smoothing, pc="cheb" Jacobi-Chebyshev alone.
+        mlp: learned patch correction after the V-cycle (flexible CG, plain fallback after
+        fallback_its iterations).  record=(snap, every): plain FP64 solve keeping iterates."""
         o = SolveOpts(int(mixed), float(rtol), int(max_it), float(inner_rtol), int(max_outer), int(cheb_degree),
-                      float(cheb_ratio), int(eig_its), {"cheb": 0, "mg": 1}[pc], int(mg_degree))
+                      float(cheb_ratio), int(eig_its), {"cheb": 0, "mg": 1}[pc], int(mg_degree), int(fallback_its))
         st = SolveStats()
-        check(_lib.dohpEllipSolve(self.h, q1.h, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(st), _stream(stream)))
-        return {f: getattr(st, f) for f, _ in SolveStats._fields_}
+        out = {}
+        if mlp is not None:
+            check(_lib.dohpEllipSolveLearned(self.h, q1.h, mlp.h, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(st),
+                                             _stream(stream)))
+        elif record is not None:
+            snap, every = record
+            ns = ctypes.c_int()
+            check(_lib.dohpEllipSolveRecord(self.h, q1.h, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(st),
+                                            _ptr(snap), int(snap.shape[0]), int(every), ctypes.byref(ns),
+                                            _stream(stream)))
+            out["nsnap"] = ns.value
+        else:
+            check(_lib.dohpEllipSolve(self.h, q1.h, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(st),
+                                      _stream(stream)))
+        out.update({f: getattr(st, f) for f, _ in SolveStats._fields_})
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):

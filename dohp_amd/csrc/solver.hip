@@ -17,6 +17,9 @@
 
 using namespace dohp;
 
+extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpEllipParam *prm, const double **kvert,
+                                              const double **state);
+
 namespace {
 
 constexpr int kRedBlocks = 1024;
@@ -164,23 +167,52 @@ struct Cheb {
   }
 };
 
-// Preconditioner of one precision: Jacobi-Chebyshev on the Q1 matrix, or a multigrid V-cycle on it.
+__global__ void scale2_kernel(long n, const double *__restrict__ s, const void *__restrict__ x, void *__restrict__ y,
+                              int f32)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (f32) ((float *)y)[i] = (float)s[i] * ((const float *)x)[i];
+  else ((double *)y)[i] = s[i] * ((const double *)x)[i];
+}
+
+// Preconditioner of one precision: Jacobi-Chebyshev on the Q1 matrix, or a multigrid V-cycle on
+// it, optionally corrected by learned node weights, z = S M(S r).
 template <typename T>
 struct Prec {
   Cheb<T> *cheb = nullptr;
   dohpMG mg = nullptr;
+  const double *s = nullptr;  // learned node weights, or null
+  T *tmp = nullptr;
+  bool learned_on = false;
   dErr apply(Ctx &c, const T *r, T *z);
 };
+template <typename T>
+dErr mg_apply(dohpMG mg, const T *r, T *z, hipStream_t st);
 template <>
-dErr Prec<double>::apply(Ctx &c, const double *r, double *z)
-{
-  return mg ? dohpMGApply(mg, r, z, c.st) : cheb->apply(c, r, z);
-}
+dErr mg_apply<double>(dohpMG mg, const double *r, double *z, hipStream_t st) { return dohpMGApply(mg, r, z, st); }
 template <>
-dErr Prec<float>::apply(Ctx &c, const float *r, float *z)
+dErr mg_apply<float>(dohpMG mg, const float *r, float *z, hipStream_t st) { return dohpMGApplyF32(mg, r, z, st); }
+template <typename T>
+dErr Prec<T>::apply(Ctx &c, const T *r, T *z)
 {
-  return mg ? dohpMGApplyF32(mg, r, z, c.st) : cheb->apply(c, r, z);
+  if (!mg) return cheb->apply(c, r, z);
+  if (!(s && learned_on)) return mg_apply<T>(mg, r, z, c.st);
+  const int f32 = sizeof(T) == 4;
+  hipLaunchKernelGGL(scale2_kernel, grid(c.n), dim3(256), 0, c.st, c.n, s, (const void *)r, (void *)tmp, f32);
+  DOHP_CALL(mg_apply<T>(mg, tmp, z, c.st));
+  hipLaunchKernelGGL(scale2_kernel, grid(c.n), dim3(256), 0, c.st, c.n, s, (const void *)z, (void *)z, f32);
+  DOHP_HIP(hipGetLastError());
+  return 0;
 }
+
+// Iteration budget shared by the sweeps of one solve: after `fallback_its` iterations with the
+// learned correction, the plain preconditioner takes over.
+struct Budget {
+  int used = 0;
+  int fallback_its = 0;
+  int fell_back = 0;
+};
 
 // Largest eigenvalue of D^{-1} A_q1 by power iteration (FP64).
 dErr estimate_lmax(Ctx &c, dohpMat q1, const double *dinv, int its, double *lmax)
@@ -211,10 +243,12 @@ dErr estimate_lmax(Ctx &c, dohpMat q1, const double *dinv, int its, double *lmax
   return 0;
 }
 
-// PCG on A x = b (x holds the initial guess).  Stops at ||r|| <= rtol ||b|| or max_it.
+// PCG on A x = b (x holds the initial guess).  Stops at ||r|| <= rtol ||b|| or max_it.  When the
+// budget switches the learned correction off, CG restarts with the plain preconditioner from the
+// current iterate.  snap (FP64 only): x after every `every`-th iteration, at most nsnap copies.
 template <typename T>
-dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, const T *b, T *x, double rtol, int max_it, int *its, double *rnorm,
-         double *bnorm)
+dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double rtol, int max_it, int *its,
+         double *rnorm, double *bnorm, T *snap = nullptr, int nsnap = 0, int every = 0, int *nsnapped = nullptr)
 {
   DevBuf<T> r, z, p, q;
   DOHP_CALL(r.alloc(c.n));
@@ -229,10 +263,20 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, const T *b, T *x, double rtol, int ma
   DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
   *bnorm = std::sqrt(bb);
   const double tol = rtol * std::sqrt(bb);
-  int k = 0;
+  int k = 0, ns = 0;
   double rz = 0;
+  bool restart = false;
+  auto precondition = [&]() -> dErr {
+    if (M.learned_on && bud.fallback_its > 0 && bud.used >= bud.fallback_its) {
+      M.learned_on = false;
+      bud.fell_back = 1;
+      restart = true;
+    }
+    return M.apply(c, r.ptr, z.ptr);
+  };
   if (std::sqrt(rr) > tol) {
-    DOHP_CALL(M.apply(c, r.ptr, z.ptr));
+    DOHP_CALL(precondition());
+    restart = false;
     DOHP_HIP(hipMemcpyAsync(p.ptr, z.ptr, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
     DOHP_CALL(dot(c, r.ptr, z.ptr, &rz));
   }
@@ -245,34 +289,50 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, const T *b, T *x, double rtol, int ma
     DOHP_CALL(axpby<T>(c, (T)-alpha, q.ptr, T(1), r.ptr));
     DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
     k++;
+    bud.used++;
+    if (snap && every > 0 && k % every == 0 && ns < nsnap) {
+      DOHP_HIP(hipMemcpyAsync(snap + (long)ns * c.n, x, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
+      ns++;
+    }
     if (std::sqrt(rr) <= tol) break;
-    DOHP_CALL(M.apply(c, r.ptr, z.ptr));
+    DOHP_CALL(precondition());
     double rz_new;
     DOHP_CALL(dot(c, r.ptr, z.ptr, &rz_new));
-    DOHP_CALL(axpby<T>(c, T(1), z.ptr, (T)(rz_new / rz), p.ptr));
+    if (restart) {  // the preconditioner changed: new Krylov sequence from here
+      DOHP_HIP(hipMemcpyAsync(p.ptr, z.ptr, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
+      restart = false;
+    } else {
+      DOHP_CALL(axpby<T>(c, T(1), z.ptr, (T)(rz_new / rz), p.ptr));
+    }
     rz = rz_new;
   }
   *its = k;
   *rnorm = std::sqrt(rr);
+  if (nsnapped) *nsnapped = ns;
   return 0;
 }
 
 }  // namespace
 
-extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double *x, const dohpSolveOpts *opts,
-                               dohpSolveStats *stats, void *stream)
+namespace {
+
+dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *x, const dohpSolveOpts *opts,
+                dohpSolveStats *stats, double *snap, int nsnap, int every, int *nsnapped, hipStream_t st)
 {
   DOHP_CHECK(op && q1 && b && x && opts && stats, dohpERR_ARG, "null argument");
+  DOHP_CHECK(opts->pc >= 0 && opts->pc <= 1, dohpERR_ARG, "pc must be 0 (Jacobi-Chebyshev) or 1 (multigrid)");
+  DOHP_CHECK(!mlp || opts->pc == 1, dohpERR_ARG, "the learned correction applies to the multigrid preconditioner");
   const auto t0 = std::chrono::steady_clock::now();
   Ctx c;
-  c.st = (hipStream_t)stream;
+  c.st = st;
   long nrows;
   DOHP_CALL(dohpMatGetSizes(q1, &nrows, nullptr));
   c.n = nrows;
   DOHP_CALL(c.part.alloc(kRedBlocks));
   DOHP_CALL(c.res.alloc(1));
   *stats = dohpSolveStats{};
-  // Jacobi scaling and the Chebyshev interval [lmax/ratio, 1.05 lmax] of D^{-1} A_q1
+  // Jacobi scaling and, for the one-level preconditioner, the Chebyshev interval
+  // [lmax/ratio, 1.05 lmax] of D^{-1} A_q1
   DevBuf<double> diag, dinv;
   DevBuf<float> dinv32;
   DOHP_CALL(diag.alloc(c.n));
@@ -280,9 +340,11 @@ extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double
   DOHP_CALL(dinv32.alloc(c.n));
   DOHP_CALL(dohpMatGetDiagonal(q1, diag.ptr, c.st));
   hipLaunchKernelGGL(inv_kernel, grid(c.n), dim3(256), 0, c.st, c.n, diag.ptr, dinv.ptr, dinv32.ptr);
-  double lmax;
-  DOHP_CALL(estimate_lmax(c, q1, dinv.ptr, opts->eig_its > 0 ? opts->eig_its : 20, &lmax));
-  lmax *= 1.05;
+  double lmax = 1.0;
+  if (opts->pc == 0) {
+    DOHP_CALL(estimate_lmax(c, q1, dinv.ptr, opts->eig_its > 0 ? opts->eig_its : 20, &lmax));
+    lmax *= 1.05;
+  }
   const double lmin = lmax / (opts->cheb_ratio > 1 ? opts->cheb_ratio : 30.0);
   stats->lmax = lmax;
   const int degree = opts->cheb_degree > 0 ? opts->cheb_degree : 4;
@@ -291,6 +353,28 @@ extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double
     ~MGHolder() { dohpMGDestroy(&mg); }
   } mgh;
   if (opts->pc == 1) DOHP_CALL(dohpMGCreate(q1, opts->mg_degree, 0, &mgh.mg, c.st));
+  Budget bud;
+  bud.fallback_its = mlp ? opts->fallback_its : 0;
+  // learned node weights (computed once: the features do not depend on the residual)
+  DevBuf<double> wts;
+  DevBuf<double> tmp64;
+  DevBuf<float> tmp32;
+  if (mlp) {
+    dohpFS fs;
+    dohpEllipParam prm;
+    const double *kv, *stt;
+    DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kv, &stt));
+    long np;
+    DOHP_CALL(dohpMLPNumPatches(q1, &np));
+    DevBuf<float> X;
+    DOHP_CALL(X.alloc((size_t)(np > 0 ? np : 1) * dohpMLPNumInputs()));
+    DOHP_CALL(wts.alloc(c.n));
+    DOHP_CALL(dohpMLPFeatures(q1, fs, dinv.ptr, X.ptr, c.st));
+    DOHP_CALL(dohpMLPWeights(mlp, q1, X.ptr, wts.ptr, nullptr, c.st));
+    DOHP_CALL(tmp64.alloc(c.n));
+    DOHP_CALL(tmp32.alloc(c.n));
+    DOHP_HIP(hipStreamSynchronize(c.st));
+  }
   const auto t_setup = std::chrono::steady_clock::now();
   stats->setup_seconds = std::chrono::duration<double>(t_setup - t0).count();
   if (!opts->mixed) {
@@ -301,9 +385,12 @@ extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double
     Prec<double> M;
     M.cheb = &C64;
     M.mg = mgh.mg;
+    M.s = wts.ptr;
+    M.tmp = tmp64.ptr;
+    M.learned_on = mlp != nullptr;
     int its;
     double rn, bn;
-    DOHP_CALL(pcg<double>(c, op, M, b, x, opts->rtol, opts->max_it, &its, &rn, &bn));
+    DOHP_CALL(pcg<double>(c, op, M, bud, b, x, opts->rtol, opts->max_it, &its, &rn, &bn, snap, nsnap, every, nsnapped));
     stats->inner_its = its;
     stats->outer_its = 1;
     stats->rnorm = rn;
@@ -316,6 +403,9 @@ extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double
     Prec<float> M;
     M.cheb = &C32;
     M.mg = mgh.mg;
+    M.s = wts.ptr;
+    M.tmp = tmp32.ptr;
+    M.learned_on = mlp != nullptr;
     DevBuf<double> r, t;
     DevBuf<float> r32, e32;
     DOHP_CALL(r.alloc(c.n));
@@ -341,7 +431,7 @@ extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double
       hipLaunchKernelGGL(fill_kernel<float>, grid(c.n), dim3(256), 0, c.st, c.n, 0.0f, e32.ptr);
       int its;
       double rn, bn;
-      DOHP_CALL(pcg<float>(c, op, M, r32.ptr, e32.ptr, opts->inner_rtol, opts->max_it, &its, &rn, &bn));
+      DOHP_CALL(pcg<float>(c, op, M, bud, r32.ptr, e32.ptr, opts->inner_rtol, opts->max_it, &its, &rn, &bn));
       inner_total += its;
       hipLaunchKernelGGL(promote_add_kernel, grid(c.n), dim3(256), 0, c.st, c.n, stats->rnorm, e32.ptr, x);
       outer++;
@@ -349,7 +439,32 @@ extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double
     stats->outer_its = outer;
     stats->inner_its = inner_total;
   }
+  stats->fallback = bud.fell_back;
   DOHP_HIP(hipStreamSynchronize(c.st));
   stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return 0;
+}
+
+}  // namespace
+
+extern "C" dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b, double *x, const dohpSolveOpts *opts,
+                               dohpSolveStats *stats, void *stream)
+{
+  return solve_impl(op, q1, nullptr, b, x, opts, stats, nullptr, 0, 0, nullptr, (hipStream_t)stream);
+}
+
+extern "C" dErr dohpEllipSolveLearned(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *x,
+                                      const dohpSolveOpts *opts, dohpSolveStats *stats, void *stream)
+{
+  DOHP_CHECK(mlp, dohpERR_ARG, "null network");
+  return solve_impl(op, q1, mlp, b, x, opts, stats, nullptr, 0, 0, nullptr, (hipStream_t)stream);
+}
+
+extern "C" dErr dohpEllipSolveRecord(dohpEllip op, dohpMat q1, const double *b, double *x, const dohpSolveOpts *opts,
+                                     dohpSolveStats *stats, double *snap_dev, int nsnap, int every, int *nsnapped,
+                                     void *stream)
+{
+  DOHP_CHECK(opts && !opts->mixed, dohpERR_ARG, "iterates are recorded by the plain FP64 solve");
+  DOHP_CHECK(snap_dev && nsnap > 0 && every > 0 && nsnapped, dohpERR_ARG, "bad snapshot arguments");
+  return solve_impl(op, q1, nullptr, b, x, opts, stats, snap_dev, nsnap, every, nsnapped, (hipStream_t)stream);
 }

@@ -151,6 +151,49 @@ dErr dohpMGApply(dohpMG mg, const double *r_dev, double *z_dev, void *stream);
 dErr dohpMGApplyF32(dohpMG mg, const float *r_dev, float *z_dev, void *stream);
 dErr dohpMGDestroy(dohpMG *mg);
 
+/* ------------------------------------------- learned correction of the Q1 preconditioner */
+/* No counterpart in the reference.  A 3-layer perceptron (32 -> 64 -> 64 -> 8, SiLU, bfloat16
+ * weights, FP32 accumulation) per disjoint 2x2x2 patch of the owned-node box maps patch features
+ * (the Q1 diagonal's log pattern and each node's position inside its element) to log-weights of
+ * the patch nodes, s = exp(clamp(o, -4, 4) / 2) (1 off the patches).  The multigrid output is
+ * corrected to z = S M(S r): linear, symmetric, positive definite, so CG stays valid.  Training
+ * minimizes the relative energy error ||e - a z||_J / ||e||_J (a one scale shared by a set of
+ * pairs) over residual/error pairs (r, e) of the operator J, with the closed-form gradient
+ * dE/ds = -2a [(J d) .* M(S r) + r .* M(S J d)] / ||e||_J^2, d = e - a z.  Parameters are one
+ * FP32 vector [W1 64x32 | b1 | W2 64x64 | b2 | W3 8x64 | b3] (row-major, dohpMLPNumParams()
+ * entries); the network uses their bfloat16 values. */
+typedef struct _p_dohpMLP *dohpMLP;
+int dohpMLPNumParams(void);
+int dohpMLPNumInputs(void);
+/* params_host NULL = all zeros (all weights 1: the plain preconditioner). */
+dErr dohpMLPCreate(const float *params_host, dohpMLP *mlp);
+/* FP32 master parameters and the bfloat16 values the network uses (either may be NULL). */
+dErr dohpMLPGetParams(dohpMLP mlp, float *master_host, float *bf16_values_host);
+dErr dohpMLPDestroy(dohpMLP *mlp);
+dErr dohpMLPNumPatches(dohpMat q1, long *npatch);
+/* Patch features X [npatch][32] of the Q1 matrix of space fs (dinv: its inverse diagonal). */
+dErr dohpMLPFeatures(dohpMat q1, dohpFS fs, const double *dinv_dev, float *X_dev, void *stream);
+/* Node weights s (FP64, and FP32 if s32_dev is not NULL) over the owned nodes. */
+dErr dohpMLPWeights(dohpMLP mlp, dohpMat q1, const float *X_dev, double *s_dev, float *s32_dev, void *stream);
+/* Mean-squared-error gradient of the raw outputs over the first B rows of X against T [B][8]
+ * (checks the training kernels). */
+dErr dohpMLPGradient(dohpMLP mlp, int B, const float *X_dev, const float *T_dev, float *grad_host, double *loss,
+                     void *stream);
+/* Energy error E = ||e - a z||_J^2 / ||e||_J^2 of z = S M(S r) for the pair (r, e) at node
+ * weights s, a = alpha if alpha > 0 else the pair's optimal scale, and its gradient dE/ds into
+ * gs_dev (may be NULL).  terms (may be NULL) receives (z, r), (z, J z), (e, J e), from which the
+ * optimal scale shared by several pairs follows. */
+dErr dohpMLPEnergyWeightGradient(dohpEllip op, dohpMG mg, long n, const double *s_dev, const double *r_dev,
+                                 const double *e_dev, double alpha, double *gs_dev, double *energy, double *terms,
+                                 void *stream);
+/* One Adam step (beta 0.9/0.999) from the gradient gs_dev = dL/ds of a loss L in the node
+ * weights (backpropagated through the network). */
+dErr dohpMLPWeightStep(dohpMLP mlp, dohpMat q1, const float *X_dev, const double *gs_dev, float lr, void *stream);
+/* One Adam step (beta 0.9/0.999) on E for the pair (r, e) with scale alpha (as above); energy
+ * receives E before the step. */
+dErr dohpMLPEnergyStep(dohpMLP mlp, dohpEllip op, dohpMat q1, dohpMG mg, const float *X_dev, const double *r_dev,
+                       const double *e_dev, double alpha, float lr, double *energy, void *stream);
+
 /* ------------------------------------------------------------------ solvers */
 typedef struct {
   int mixed;          /* 0: plain FP64 PCG; 1: FP32 inner PCG inside FP64 defect correction */
@@ -163,12 +206,14 @@ typedef struct {
   int eig_its;        /* power iterations for lmax */
   int pc;             /* 0: Jacobi-Chebyshev on the Q1 matrix; 1: multigrid V-cycle on it (dohpMG) */
   int mg_degree;      /* Chebyshev degree of the multigrid smoother (pre and post) */
+  int fallback_its;   /* learned solve: iterations after which the plain preconditioner takes over (0 = never) */
 } dohpSolveOpts;
 
 typedef struct {
   int outer_its, inner_its;
   double bnorm, rnorm, lmax;
   double setup_seconds, seconds;
+  int fallback;       /* learned solve: 1 if the plain preconditioner took over */
 } dohpSolveStats;
 
 /* Solve J(u) x = b (the linear operator of the ellip context; x holds the initial guess) by CG
@@ -177,6 +222,16 @@ typedef struct {
  * outer residual accumulate in FP64. */
 dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b_dev, double *x_dev, const dohpSolveOpts *opts,
                     dohpSolveStats *stats, void *stream);
+/* The same with the multigrid preconditioner corrected by the learned node weights (no
+ * counterpart in the reference): z = S M(S r).  After opts->fallback_its iterations without
+ * convergence the plain multigrid preconditioner takes over (CG restarts from the current
+ * iterate). */
+dErr dohpEllipSolveLearned(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b_dev, double *x_dev,
+                           const dohpSolveOpts *opts, dohpSolveStats *stats, void *stream);
+/* Plain FP64 solve that also records the iterate after every `every`-th iteration into
+ * snap_dev [nsnap][nowned] (the residual/error pairs of the learned correction's training). */
+dErr dohpEllipSolveRecord(dohpEllip op, dohpMat q1, const double *b_dev, double *x_dev, const dohpSolveOpts *opts,
+                          dohpSolveStats *stats, double *snap_dev, int nsnap, int every, int *nsnapped, void *stream);
 
 #ifdef __cplusplus
 }

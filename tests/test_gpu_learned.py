@@ -1,0 +1,115 @@
+"""Learned node weights of the Q1 multigrid preconditioner: HIP features, weights and training
+gradients against the numpy oracle and finite differences; training lowers the energy error of
+held-out pairs; learned solves converge to the FP64 tolerance; the fallback engages."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import mlp_oracle as L  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def C():
+    assert torch.cuda.is_available()
+    import dohp_amd.capi as capi
+    return capi
+
+
+@pytest.fixture(scope="module")
+def small(C):
+    from dohp_amd.learned import make_problem
+    return make_problem(C, (4, 4, 3), 4, field_seed=3, torch=torch)
+
+
+def _t(a, dt=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+
+
+def test_gradient_matches_oracle(C):
+    rng = np.random.default_rng(0)
+    w = (rng.standard_normal(L.NPARAM) * 0.2).astype(np.float32)
+    B = 300
+    X = rng.standard_normal((B, L.NIN)).astype(np.float32)
+    T = rng.standard_normal((B, L.NOUT)).astype(np.float32)
+    net = C.MLP(w)
+    loss, g = net.gradient(_t(X, torch.float32), _t(T, torch.float32))
+    lref, gref = L.loss_and_grad(L.bf16(w), X.astype(np.float64), T.astype(np.float64))
+    print("loss", loss, lref, "grad rel err", np.linalg.norm(g - gref) / np.linalg.norm(gref))
+    assert abs(loss - lref) <= 1e-5 * lref
+    assert np.linalg.norm(g - gref) <= 1e-5 * np.linalg.norm(gref)
+    m, wb = net.params()
+    np.testing.assert_array_equal(m, w)
+    np.testing.assert_array_equal(wb, L.bf16(w))
+
+
+def test_features_and_weights_match_oracle(C, small):
+    fs, op, q1, b, xs = small
+    box = C.MG(q1).level(0)[0]
+    n = fs.nowned
+    diag = torch.zeros(n, dtype=torch.float64, device="cuda")
+    q1.diagonal(diag)
+    dinv = (1.0 / diag).cpu().numpy()
+    npatch = C.MLP.num_patches(q1)
+    X = torch.zeros((npatch, L.NIN), dtype=torch.float32, device="cuda")
+    C.MLP.features(q1, fs, _t(dinv), X)
+    Xr = L.features(box, 4, (1, 1, 1), dinv)  # Dirichlet on every face: the owned box starts at lattice 1
+    np.testing.assert_allclose(X.cpu().numpy(), Xr, rtol=1e-5, atol=1e-5)
+    w = (np.random.default_rng(1).standard_normal(L.NPARAM) * 0.3).astype(np.float32)
+    net = C.MLP(w)
+    s = torch.zeros(n, dtype=torch.float64, device="cuda")
+    net.weights(q1, X, s)
+    sr = L.node_weights(box, w, X.cpu().numpy().astype(np.float64))
+    np.testing.assert_allclose(s.cpu().numpy(), sr, rtol=2e-5)
+    assert float(s.min()) > 0
+
+
+def test_energy_gradient_matches_finite_differences(C, small):
+    fs, op, q1, b, xs = small
+    mg = C.MG(q1)
+    n = fs.nowned
+    rng = np.random.default_rng(4)
+    e = _t(rng.standard_normal(n))
+    r = torch.zeros_like(e)
+    op.mat_mult(e, r)
+    s = _t(np.exp(0.3 * rng.standard_normal(n)))
+    for alpha in (0.0, 0.05):  # the pair's optimal scale (stationary), and a fixed one
+        gs = torch.zeros_like(s)
+        E0, terms = C.MLP.energy_weight_gradient(op, mg, s, r, e, alpha, gs)
+        assert E0 > 0
+        for k in range(3):
+            v = _t(rng.standard_normal(n))
+            h = 1e-5
+            a = alpha if alpha > 0 else 0.0
+            Ep, _ = C.MLP.energy_weight_gradient(op, mg, s + h * v, r, e, a)
+            Em, _ = C.MLP.energy_weight_gradient(op, mg, s - h * v, r, e, a)
+            fd, an = (Ep - Em) / (2 * h), float(torch.dot(gs, v))
+            print("alpha", alpha, "directional derivative", fd, an)
+            assert abs(fd - an) <= 1e-5 * (abs(fd) + 1e-3 * E0)
+
+
+def test_training_and_learned_solves(C, small):
+    from dohp_amd.learned import fit, init_params
+    net, rep = fit(C, torch, train_seeds=[11, 12], val_seeds=[21], M=(6, 6, 6), deg=4, epochs=8, lr=3e-3,
+                   lanczos=20)
+    print({k: v for k, v in rep.items() if k != "history"})
+    for h in rep["history"]:
+        print(h)
+    # the spectral spread of the training problems' preconditioned operators decreases
+    assert rep["history"][-1]["train_log_spread"] < rep["history"][0]["train_log_spread"]
+    if net is None:  # no epoch beat the plain preconditioner on the validation problem
+        net = C.MLP(init_params(C.MLP.nparam))
+    fs, op, q1, b, xs = small
+    res = {}
+    for name, kw in [("plain", {}), ("learned", {"mlp": net}), ("learned_mixed", {"mlp": net, "mixed": True}),
+                     ("fallback", {"mlp": net, "fallback_its": 2})]:
+        x = torch.zeros_like(b)
+        st = op.solve(q1, b, x, rtol=1e-10, pc="mg", **kw)
+        r = torch.zeros_like(b)
+        op.mat_mult(x, r)
+        relres = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
+        print(name, st["inner_its"], st["outer_its"], st["fallback"], relres)
+        assert relres <= 1.01e-10
+        res[name] = st
+    assert res["fallback"]["fallback"] == 1 and res["learned"]["fallback"] == 0

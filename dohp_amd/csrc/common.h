@@ -114,6 +114,11 @@ struct _p_dohpMat {
   int box[3] = {0, 0, 0};
   dohp::DevBuf<double> st64;
   dohp::DevBuf<float> st32;
+  // multigrid hierarchy built by the first solve with a given smoothing degree; dropped when the
+  // matrix is refilled
+  dohpMG mg = nullptr;
+  int mg_degree = 0;
+  ~_p_dohpMat() { dohpMGDestroy(&mg); }
 };
 dErr dohpMatBuildStencil_Private(_p_dohpMat *A, dohpFS fs, hipStream_t st);
 dErr dohpStencilMult_Private(const _p_dohpMat *A, const double *x, double *y, hipStream_t st);

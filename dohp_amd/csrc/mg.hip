@@ -8,7 +8,8 @@
 // (missing neighbours are Dirichlet zeros).  Coarse operators are Galerkin products P^T A P
 // (again 27-point), smoothing is a Jacobi-preconditioned Chebyshev polynomial on
 // [lo * lmax, hi * lmax] of D^{-1} A, the coarsest level (at most a few hundred nodes) is solved
-// with its explicit inverse.  Pre- and post-smoothers are the same polynomial, so the V-cycle is
+// with its explicit inverse.  The V-cycle works on symmetric half stencils (see
+// half_residual_kernel).  Pre- and post-smoothers are the same polynomial, so the V-cycle is
 // a symmetric operator and serves as a CG preconditioner.
 #include <cmath>
 #include <vector>
@@ -23,6 +24,24 @@ struct Box {
   int nx, ny, nz;
   __host__ __device__ long n() const { return (long)nx * ny * nz; }
 };
+
+// Box kernels launch a 3-D grid: x covers the last (contiguous) index K in chunks of 64, y the
+// middle index J in chunks of 4 (threadIdx.y), z the first index I; no index division.
+constexpr int kBX = 64, kBY = 4;
+inline dim3 box_grid(const Box &b)
+{
+  return dim3((unsigned)((b.nz + kBX - 1) / kBX), (unsigned)((b.ny + kBY - 1) / kBY), (unsigned)b.nx);
+}
+inline dim3 box_block() { return dim3(kBX, kBY); }
+__device__ __forceinline__ bool box_index(const Box &b, int &I, int &J, int &K, long &r)
+{
+  K = blockIdx.x * kBX + threadIdx.x;
+  J = blockIdx.y * kBY + threadIdx.y;
+  I = blockIdx.z;
+  if (K >= b.nz || J >= b.ny) return false;
+  r = ((long)I * b.ny + J) * b.nz + K;
+  return true;
+}
 
 __device__ __forceinline__ void unrank(const Box &b, long r, int &I, int &J, int &K)
 {
@@ -63,10 +82,9 @@ template <typename T>
 __global__ void stencil_mult_kernel(Box b, const T *__restrict__ A, const T *__restrict__ x, T *__restrict__ y)
 {
   const long n = b.n();
-  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
   int I, J, K;
-  unrank(b, r, I, J, K);
+  long r;
+  if (!box_index(b, I, J, K, r)) return;
   const long sy = b.nz, sx = (long)b.ny * b.nz;
   T acc = 0;
 #pragma unroll
@@ -78,25 +96,41 @@ __global__ void stencil_mult_kernel(Box b, const T *__restrict__ A, const T *__r
   y[r] = acc;
 }
 
-// r = b - A x  (b may be null: r = -A x)
+// The Q1 matrix and its Galerkin coarse operators are symmetric, so the levels keep only the
+// diagonal and the 13 upper neighbours, H[s - 13][r] for s = 13..26; the entry of a lower
+// neighbour is the upper entry stored at that neighbour's row.  The product reads 14 coefficient
+// streams instead of 27 (the shifted reads of the lower half hit the caches).
 template <typename T>
-__global__ void stencil_residual_kernel(Box bx, const T *__restrict__ A, const T *__restrict__ b,
-                                        const T *__restrict__ x, T *__restrict__ r)
+__global__ void __launch_bounds__(256) half_residual_kernel(Box bx, const T *__restrict__ H, const T *__restrict__ b,
+                                                            const T *__restrict__ x, T *__restrict__ r)
 {
   const long n = bx.n();
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   int I, J, K;
-  unrank(bx, i, I, J, K);
+  long i;
+  if (!box_index(bx, I, J, K, i)) return;
   const long sy = bx.nz, sx = (long)bx.ny * bx.nz;
-  T acc = 0;
+  T acc = H[i] * x[i];
 #pragma unroll
-  for (int s = 0; s < 27; s++) {
+  for (int s = 14; s < 27; s++) {
     const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
-    if (I + di < 0 || I + di >= bx.nx || J + dj < 0 || J + dj >= bx.ny || K + dk < 0 || K + dk >= bx.nz) continue;
-    acc += A[s * n + i] * x[i + di * sx + dj * sy + dk];
+    const long off = di * sx + dj * sy + dk;
+    if (I + di >= 0 && I + di < bx.nx && J + dj >= 0 && J + dj < bx.ny && K + dk >= 0 && K + dk < bx.nz)
+      acc += H[(s - 13) * n + i] * x[i + off];
+    if (I - di >= 0 && I - di < bx.nx && J - dj >= 0 && J - dj < bx.ny && K - dk >= 0 && K - dk < bx.nz)
+      acc += H[(s - 13) * n + i - off] * x[i - off];
   }
   r[i] = (b ? b[i] : T(0)) - acc;
+}
+
+__global__ void half_from_full_kernel(long n, const double *__restrict__ A, double *__restrict__ H,
+                                      float *__restrict__ H32)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int s = 13; s < 27; s++) {
+    H[(s - 13) * n + i] = A[s * n + i];
+    H32[(s - 13) * n + i] = (float)A[s * n + i];
+  }
 }
 
 template <typename T>
@@ -137,10 +171,9 @@ __device__ __forceinline__ int parents(int f, int nc, int (&c)[2], double (&w)[2
 template <typename T>
 __global__ void restrict_kernel(Box f, Box c, const T *__restrict__ rf, T *__restrict__ bc)
 {
-  const long C = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (C >= c.n()) return;
   int I, J, K;
-  unrank(c, C, I, J, K);
+  long C;
+  if (!box_index(c, I, J, K, C)) return;
   const long sy = f.nz, sx = (long)f.ny * f.nz;
   const int F0 = 2 * I + 1, F1 = 2 * J + 1, F2 = 2 * K + 1;
   T acc = 0;
@@ -162,10 +195,9 @@ __global__ void restrict_kernel(Box f, Box c, const T *__restrict__ rf, T *__res
 template <typename T>
 __global__ void prolong_add_kernel(Box f, Box c, const T *__restrict__ xc, T *__restrict__ xf)
 {
-  const long F = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (F >= f.n()) return;
   int I, J, K;
-  unrank(f, F, I, J, K);
+  long F;
+  if (!box_index(f, I, J, K, F)) return;
   int ci[2], cj[2], ck[2];
   double wi[2], wj[2], wk[2];
   const int mi = parents(I, c.nx, ci, wi), mj = parents(J, c.ny, cj, wj), mk = parents(K, c.nz, ck, wk);
@@ -220,14 +252,17 @@ __global__ void rap_kernel(Box f, Box c, const double *__restrict__ Af, double *
   for (int s = 0; s < 27; s++) Ac[s * nc + C] = acc[s];
 }
 
+// x = Ainv b, one wavefront per row (lanes stride the row, then a butterfly reduction).
 template <typename T>
 __global__ void dense_mv_kernel(int n, const T *__restrict__ Ainv, const T *__restrict__ b, T *__restrict__ x)
 {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+  const int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x & 63;
+  if (r >= n) return;  // whole wavefronts exit together
   T acc = 0;
-  for (int c = 0; c < n; c++) acc += Ainv[(long)r * n + c] * b[c];
-  x[r] = acc;
+  for (int c = lane; c < n; c += 64) acc += Ainv[(long)r * n + c] * b[c];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) x[r] = acc;
 }
 
 template <typename T>
@@ -284,7 +319,7 @@ dErr dohpMatBuildStencil_Private(_p_dohpMat *A, dohpFS fs, hipStream_t st)
 dErr dohpStencilMult_Private(const _p_dohpMat *A, const double *x, double *y, hipStream_t st)
 {
   const Box b{A->box[0], A->box[1], A->box[2]};
-  hipLaunchKernelGGL(stencil_mult_kernel<double>, grid_of(b.n()), dim3(256), 0, st, b, A->st64.ptr, x, y);
+  hipLaunchKernelGGL(stencil_mult_kernel<double>, box_grid(b), box_block(), 0, st, b, A->st64.ptr, x, y);
   DOHP_HIP(hipGetLastError());
   return 0;
 }
@@ -292,7 +327,7 @@ dErr dohpStencilMult_Private(const _p_dohpMat *A, const double *x, double *y, hi
 dErr dohpStencilMultF32_Private(const _p_dohpMat *A, const float *x, float *y, hipStream_t st)
 {
   const Box b{A->box[0], A->box[1], A->box[2]};
-  hipLaunchKernelGGL(stencil_mult_kernel<float>, grid_of(b.n()), dim3(256), 0, st, b, A->st32.ptr, x, y);
+  hipLaunchKernelGGL(stencil_mult_kernel<float>, box_grid(b), box_block(), 0, st, b, A->st32.ptr, x, y);
   DOHP_HIP(hipGetLastError());
   return 0;
 }
@@ -300,10 +335,9 @@ dErr dohpStencilMultF32_Private(const _p_dohpMat *A, const float *x, float *y, h
 struct MGLevel {
   Box box{};
   long n = 0;
-  const double *A64 = nullptr;  // finest level: the Q1 matrix's own stencil
-  const float *A32 = nullptr;
-  DevBuf<double> own64, dinv64, b64, x64, r64, d64;
-  DevBuf<float> own32, dinv32, b32, x32, r32, d32;
+  const double *A64 = nullptr;  // full 27-point stencil (finest level: the Q1 matrix's own)
+  DevBuf<double> own64, H64, dinv64, b64, x64, r64, d64;  // H: symmetric half stencil [14][n]
+  DevBuf<float> H32, dinv32, b32, x32, r32, d32;
   double lmax = 0;
 };
 
@@ -347,8 +381,9 @@ dErr estimate_lmax(MGLevel &lv, hipStream_t st, int its)
   DOHP_CALL(norm(v.ptr, &nv));
   hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0 / nv, nullptr, v.ptr, v.ptr);
   for (int k = 0; k < its; k++) {
-    hipLaunchKernelGGL(stencil_mult_kernel<double>, grid_of(lv.n), dim3(256), 0, st, lv.box, lv.A64, v.ptr, w.ptr);
-    hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0, lv.dinv64.ptr, w.ptr, w.ptr);
+    hipLaunchKernelGGL(half_residual_kernel<double>, box_grid(lv.box), box_block(), 0, st, lv.box, lv.H64.ptr,
+                       (const double *)nullptr, v.ptr, w.ptr);  // w = -A v
+    hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, -1.0, lv.dinv64.ptr, w.ptr, w.ptr);
     DOHP_CALL(norm(w.ptr, &nw));
     hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0 / nw, nullptr, w.ptr, v.ptr);
   }
@@ -402,7 +437,7 @@ template <typename T>
 struct LvPtr;
 template <>
 struct LvPtr<double> {
-  static const double *A(const MGLevel &l) { return l.A64; }
+  static const double *A(const MGLevel &l) { return l.H64.ptr; }
   static const double *dinv(const MGLevel &l) { return l.dinv64.ptr; }
   static double *b(MGLevel &l) { return l.b64.ptr; }
   static double *x(MGLevel &l) { return l.x64.ptr; }
@@ -412,7 +447,7 @@ struct LvPtr<double> {
 };
 template <>
 struct LvPtr<float> {
-  static const float *A(const MGLevel &l) { return l.A32; }
+  static const float *A(const MGLevel &l) { return l.H32.ptr; }
   static const float *dinv(const MGLevel &l) { return l.dinv32.ptr; }
   static float *b(MGLevel &l) { return l.b32.ptr; }
   static float *x(MGLevel &l) { return l.x32.ptr; }
@@ -434,11 +469,11 @@ void smooth(const _p_dohpMG *mg, MGLevel &lv, const T *b, T *x, bool zero_init, 
   if (zero_init)
     hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, T(0), (T)(1.0 / theta), P::dinv(lv), b, d, x, 1);
   else {
-    hipLaunchKernelGGL(stencil_residual_kernel<T>, g, dim3(256), 0, st, lv.box, P::A(lv), b, x, r);
+    hipLaunchKernelGGL(half_residual_kernel<T>, box_grid(lv.box), box_block(), 0, st, lv.box, P::A(lv), b, x, r);
     hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, T(0), (T)(1.0 / theta), P::dinv(lv), r, d, x, 0);
   }
   for (int k = 1; k < mg->degree; k++) {
-    hipLaunchKernelGGL(stencil_residual_kernel<T>, g, dim3(256), 0, st, lv.box, P::A(lv), b, x, r);
+    hipLaunchKernelGGL(half_residual_kernel<T>, box_grid(lv.box), box_block(), 0, st, lv.box, P::A(lv), b, x, r);
     const double rho_new = 1.0 / (2.0 * sigma - rho);
     hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, (T)(rho_new * rho), (T)(2.0 * rho_new / delta),
                        P::dinv(lv), r, d, x, 0);
@@ -452,16 +487,16 @@ void vcycle(_p_dohpMG *mg, size_t l, const T *b, T *x, hipStream_t st)
   using P = LvPtr<T>;
   MGLevel &lv = mg->L[l];
   if (l + 1 == mg->L.size()) {
-    hipLaunchKernelGGL(dense_mv_kernel<T>, dim3((unsigned)((lv.n + 127) / 128)), dim3(128), 0, st, (int)lv.n,
+    hipLaunchKernelGGL(dense_mv_kernel<T>, dim3((unsigned)((lv.n + 3) / 4)), dim3(256), 0, st, (int)lv.n,
                        P::cinv(mg), b, x);
     return;
   }
   MGLevel &cv = mg->L[l + 1];
   smooth<T>(mg, lv, b, x, true, st);
-  hipLaunchKernelGGL(stencil_residual_kernel<T>, grid_of(lv.n), dim3(256), 0, st, lv.box, P::A(lv), b, x, P::r(lv));
-  hipLaunchKernelGGL(restrict_kernel<T>, grid_of(cv.n), dim3(256), 0, st, lv.box, cv.box, P::r(lv), P::b(cv));
+  hipLaunchKernelGGL(half_residual_kernel<T>, box_grid(lv.box), box_block(), 0, st, lv.box, P::A(lv), b, x, P::r(lv));
+  hipLaunchKernelGGL(restrict_kernel<T>, box_grid(cv.box), box_block(), 0, st, lv.box, cv.box, P::r(lv), P::b(cv));
   vcycle<T>(mg, l + 1, P::b(cv), P::x(cv), st);
-  hipLaunchKernelGGL(prolong_add_kernel<T>, grid_of(lv.n), dim3(256), 0, st, lv.box, cv.box, P::x(cv), x);
+  hipLaunchKernelGGL(prolong_add_kernel<T>, box_grid(lv.box), box_block(), 0, st, lv.box, cv.box, P::x(cv), x);
   smooth<T>(mg, lv, b, x, false, st);
 }
 
@@ -482,7 +517,6 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
     lv.box = Box{q1->box[0], q1->box[1], q1->box[2]};
     lv.n = lv.box.n();
     lv.A64 = q1->st64.ptr;
-    lv.A32 = q1->st32.ptr;
     mg->L.push_back(std::move(lv));
   }
   while (!err) {
@@ -494,12 +528,9 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
     c.box = cb;
     c.n = cb.n();
     err = c.own64.alloc(27 * c.n);
-    if (!err) err = c.own32.alloc(27 * c.n);
     if (err) break;
     hipLaunchKernelGGL(rap_kernel, grid_of(c.n), dim3(256), 0, st, f.box, cb, f.A64, c.own64.ptr);
-    hipLaunchKernelGGL(to_f32<float>, grid_of(27 * c.n), dim3(256), 0, st, 27 * c.n, c.own64.ptr, c.own32.ptr);
     c.A64 = c.own64.ptr;
-    c.A32 = c.own32.ptr;
     mg->L.push_back(std::move(c));
   }
   for (size_t l = 0; !err && l < mg->L.size(); l++) {
@@ -507,6 +538,8 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
     const long n = lv.n;
     err = lv.dinv64.alloc(n);
     if (!err) err = lv.dinv32.alloc(n);
+    if (!err) err = lv.H64.alloc(14 * n);
+    if (!err) err = lv.H32.alloc(14 * n);
     if (!err) err = lv.r64.alloc(n);
     if (!err) err = lv.d64.alloc(n);
     if (!err) err = lv.r32.alloc(n);
@@ -516,6 +549,7 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
     if (!err && l > 0) err = lv.b32.alloc(n);
     if (!err && l > 0) err = lv.x32.alloc(n);
     if (err) break;
+    hipLaunchKernelGGL(half_from_full_kernel, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.H64.ptr, lv.H32.ptr);
     hipLaunchKernelGGL(diag_inv_kernel<double>, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.dinv64.ptr);
     hipLaunchKernelGGL(to_f32<float>, grid_of(n), dim3(256), 0, st, n, lv.dinv64.ptr, lv.dinv32.ptr);
     if (l + 1 < mg->L.size()) err = estimate_lmax(lv, st, 15);

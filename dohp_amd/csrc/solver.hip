@@ -9,6 +9,7 @@
 //   * Mixed path: FP64 outer defect correction x += A32^{-1}(b - A64 x) with the inner PCG in
 //     FP32 (FP32 operator and Q1 applies); residuals of the outer loop and every Krylov dot
 //     product accumulate in FP64.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -348,11 +349,18 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
   const double lmin = lmax / (opts->cheb_ratio > 1 ? opts->cheb_ratio : 30.0);
   stats->lmax = lmax;
   const int degree = opts->cheb_degree > 0 ? opts->cheb_degree : 4;
-  struct MGHolder {
+  struct {
     dohpMG mg = nullptr;
-    ~MGHolder() { dohpMGDestroy(&mg); }
   } mgh;
-  if (opts->pc == 1) DOHP_CALL(dohpMGCreate(q1, opts->mg_degree, 0, &mgh.mg, c.st));
+  if (opts->pc == 1) {  // the hierarchy is kept with the matrix until it is refilled
+    const int deg = opts->mg_degree > 0 ? opts->mg_degree : 2;
+    if (!q1->mg || q1->mg_degree != deg) {
+      dohpMGDestroy(&q1->mg);
+      DOHP_CALL(dohpMGCreate(q1, deg, 0, &q1->mg, c.st));
+      q1->mg_degree = deg;
+    }
+    mgh.mg = q1->mg;
+  }
   Budget bud;
   bud.fallback_its = mlp ? opts->fallback_its : 0;
   // learned node weights (computed once: the features do not depend on the residual)
@@ -431,7 +439,10 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
       hipLaunchKernelGGL(fill_kernel<float>, grid(c.n), dim3(256), 0, c.st, c.n, 0.0f, e32.ptr);
       int its;
       double rn, bn;
-      DOHP_CALL(pcg<float>(c, op, M, bud, r32.ptr, e32.ptr, opts->inner_rtol, opts->max_it, &its, &rn, &bn));
+      // the last sweep needs only the reduction that is left to reach rtol ||b||
+      const double need = 0.5 * opts->rtol * stats->bnorm / stats->rnorm;
+      const double irtol = std::max(opts->inner_rtol, need);
+      DOHP_CALL(pcg<float>(c, op, M, bud, r32.ptr, e32.ptr, irtol, opts->max_it, &its, &rn, &bn));
       inner_total += its;
       hipLaunchKernelGGL(promote_add_kernel, grid(c.n), dim3(256), 0, c.st, c.n, stats->rnorm, e32.ptr, x);
       outer++;

@@ -219,7 +219,8 @@ typedef struct {
 /* Solve J(u) x = b (the linear operator of the ellip context; x holds the initial guess) by CG
  * preconditioned with the embedded-Q1 matrix under Jacobi-Chebyshev smoothing -- the KSP/PC
  * role of ellip's driver (src/fs/tests/ellip.c:594-652,701-747).  Krylov dot products and the
- * outer residual accumulate in FP64. */
+ * outer residual accumulate in FP64.  With pc = 1 the multigrid hierarchy is built by the first
+ * solve and kept with q1 until q1 is refilled (stats->setup_seconds includes it when built). */
 dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b_dev, double *x_dev, const dohpSolveOpts *opts,
                     dohpSolveStats *stats, void *stream);
 /* The same with the multigrid preconditioner corrected by the learned node weights (no

@@ -307,6 +307,8 @@ def main():
     torch.cuda.synchronize()
     variants["jv_f32"]["rel_err_vs_f64"] = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
 
+    variants["tuning"] = {f"{m}_{'f32' if f else 'f64'}": (lin if m == "linear" else op).tuning(m, f)
+                          for m, f in (("jacobian", 0), ("jacobian", 1), ("residual", 0), ("linear", 0), ("linear", 1))}
     B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True)
     achieved = B / (t_kernel or t_step) / 1e9
     workload = f"nonlinear diffusion J(u0)v, {Me}^3 hex, p={p}, FP64, perturbed mesh"

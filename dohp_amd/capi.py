@@ -66,6 +66,7 @@ _proto("dohpEllipShellMatMult", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMultF32", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMultAdd", _vp, _vp, _vp, _vp)
 _proto("dohpEllipSetScatter", _vp, ctypes.c_int)
+_proto("dohpEllipGetTuning", _vp, ctypes.c_int, ctypes.c_int, _vp, _vp)
 _proto("dohpEllipAssembleQ1", _vp, _vp, ctypes.POINTER(_vp), _vp)
 _proto("dohpMatGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long))
 _proto("dohpMatGetCSR", _vp, _ip, _ip, _dp)
@@ -375,6 +376,13 @@ class Ellip:
         else:
             k, kp = _np(kvert)
             check(_lib.dohpEllipSetCoefficient(self.h, kp))
+
+    def tuning(self, mode, fp32=False):
+        """(chosen shape, ms of [four-wavefront, one-wavefront, persistent one-wavefront])."""
+        sh, ms = ctypes.c_int(), (ctypes.c_float * 3)()
+        check(_lib.dohpEllipGetTuning(self.h, {"linear": 0, "residual": 1, "jacobian": 2}[mode], int(fp32),
+                                      ctypes.byref(sh), ms))
+        return sh.value, [round(v, 4) for v in ms]
 
     def set_scatter(self, mode):
         check(_lib.dohpEllipSetScatter(self.h, int(mode)))

@@ -22,11 +22,6 @@ namespace {
 
 enum { MODE_LIN = 0, MODE_FUN = 1, MODE_JAC = 2 };
 
-template <int N, typename T>
-struct Tab {
-  double w[N];   // Gauss weights
-  double phi[N]; // (1 + z_q)/2
-};
 
 struct Phys {
   double p, eps2, lambda, ea, eb, ec;
@@ -35,7 +30,10 @@ struct Phys {
   int qpow;  // 4 (p-2)/2 when it is an integer in [-8, 8], else 99 (general power)
 };
 
-constexpr int elems_per_block(int N) { return (256 / (N * N)) > 0 ? 256 / (N * N) : 1; }
+// Workgroup shapes the tuner chooses from: elements per workgroup filling four wavefronts, or one
+// wavefront (one element per wavefront from n = 6 up; two wavefronts for n >= 9).
+constexpr int epb_wide(int N) { return (256 / (N * N)) > 0 ? 256 / (N * N) : 1; }
+constexpr int epb_narrow(int N) { return (64 / (N * N)) > 0 ? 64 / (N * N) : 1; }
 
 // Exact-solution forcing f = -div(eta grad u) (ellip.c:42-117), from u's gradient and Hessian.
 __device__ double forcing(const Phys &ph, double x, double y, double z)
@@ -336,10 +334,10 @@ __device__ inline T sqrt_pos(T x) { return x > T(0) ? x * rsqrt_nr(x) : sqrt(x);
 // Column geometry of the trilinear map along the z line (xi0 = z_a, xi1 = z_b) of an element
 // whose 8 vertices (x, y, z, k) sit at X[32]; also the line's coordinate and coefficient ends.
 template <int N, typename T>
-__device__ __forceinline__ ColGeo<T> column_geometry(const double *X, const Tab<N, T> &tb, int a, int b,
+__device__ __forceinline__ ColGeo<T> column_geometry(const double *X, const double *phi, int a, int b,
                                                      double (&Pc)[2][3], double (&Kc)[2])
 {
-  const double fa1 = tb.phi[a], fa0 = 1.0 - fa1, fb1 = tb.phi[b], fb0 = 1.0 - fb1;
+  const double fa1 = phi[a], fa0 = 1.0 - fa1, fb1 = phi[b], fb0 = 1.0 - fb1;
   double Rc[2][3], Sc[2][3];
 #pragma unroll
   for (int c = 0; c < 2; c++) {
@@ -361,34 +359,43 @@ __device__ __forceinline__ ColGeo<T> column_geometry(const double *X, const Tab<
   return make_colgeo<T>(Rc, Sc, c2);
 }
 
-template <int N, typename T, int MODE>
-__global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elist, long ne,
+// Threads of a workgroup holding EPB elements: one per line, EPB * n^2.  A workgroup smaller than a
+// wavefront is rounded up to one: its extra lanes repeat lines of the same wavefront, which runs in
+// lockstep, so they load, compute and store the same values at the same time as the lines' owners.
+// (Repeating lines of another wavefront would race with in-place line updates.)
+constexpr int wg_threads(int N, int EPB) { return EPB * N * N < 64 ? 64 : EPB * N * N; }
+
+
+template <int N, typename T, int MODE, int EPB>
+__global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__restrict__ elist, long ne,
                                                     const int *__restrict__ enodes, const int *__restrict__ everts,
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
                                                     const T *__restrict__ ustate, const T *__restrict__ xdir,
                                                     T *__restrict__ y, long nowned, int atomic,
-                                                    const T *__restrict__ gtab, Tab<N, T> tb, Phys ph)
+                                                    const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph)
 {
-  constexpr int NN = N * N, N3 = NN * N, EPB = elems_per_block(N), NTH = EPB * NN;
+  constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
   __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
   __shared__ int IDX[EPB * N3];
   __shared__ double VX[EPB * 8 * 4];
 
   const long lb = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
-  const int le = tid / NN, ab = tid - le * NN, a = ab / N, b = ab - a * N;
+  const int line = tid % (EPB * NN), le = line / NN, ab = line - le * NN, a = ab / N, b = ab - a * N;
 #pragma unroll
-  for (int s = 0; s < N; s++) {
-    const int t = s * NTH + tid, l = t / N3;
-    const long es = lb * EPB + l;
-    int v = -1;
-    if (es < ne) {
-      const long e = elist ? elist[es] : es;
-      v = enodes[e * N3 + (t - l * N3)];
+  for (int s = 0; s < NS; s++) {
+    const int t = s * NT + tid, l = t / N3;
+    if (t < EPB * N3) {
+      const long es = lb * EPB + l;
+      int v = -1;
+      if (es < ne) {
+        const long e = elist ? elist[es] : es;
+        v = enodes[e * N3 + (t - l * N3)];
+      }
+      IDX[t] = v;
     }
-    IDX[t] = v;
   }
-  for (int t = tid; t < EPB * 8; t += NTH) {
+  for (int t = tid; t < EPB * 8; t += NT) {
     const int l = t >> 3, vt = t & 7;
     const long es = lb * EPB + l;
     double *dst = &VX[(l * 8 + vt) * 4];
@@ -409,136 +416,126 @@ __global__ void __launch_bounds__(256) ellip_kernel(const int *__restrict__ elis
   __syncthreads();
   // gather of the first field, coalesced over the element node lists (tile order = node order);
   // the Jacobian action prefetches the state into registers behind the direction's contractions
-  T upre[MODE == MODE_JAC ? N : 1];
+  T upre[MODE == MODE_JAC ? NS : 1];
 #pragma unroll
-  for (int s = 0; s < N; s++) {
-    const int t = s * NTH + tid, c = IDX[t];
-    T v = 0;
-    if (MODE == MODE_FUN) v = c >= 0 ? ustate[c] : T(0);
-    else v = (c >= 0 && c < nowned) ? xdir[c] : T(0);
-    TA[t] = v;
-    if (MODE == MODE_JAC) upre[s] = c >= 0 ? ustate[c] : T(0);
+  for (int s = 0; s < NS; s++) {
+    const int t = s * NT + tid;
+    if (t < EPB * N3) {
+      const int c = IDX[t];
+      T v = 0;
+      if (MODE == MODE_FUN) v = c >= 0 ? ustate[c] : T(0);
+      else v = (c >= 0 && c < nowned) ? xdir[c] : T(0);
+      TA[t] = v;
+      if (MODE == MODE_JAC) upre[s] = c >= 0 ? ustate[c] : T(0);
+    }
   }
   __syncthreads();
-  T *A = &TA[le * N3], *GX = &TGX[le * N3], *GY = &TGY[le * N3];
+  int tid_body = tid;  // derived values are formed here, not hoisted above the gather
+  asm volatile("" : "+v"(tid_body));
+#define DOHP_BATCH_HOOK() ((void)0)
+#include "ellip_batch.inc"
+#undef DOHP_BATCH_HOOK
+}
 
-  // geometry is formed after the forward contractions so that it does not occupy registers there
-  const T wij = (T)(tb.w[a] * tb.w[b]);
-  double Pc[2][3], Kc[2];
-  T vq[N], d0[N], d1[N], d2[N];
-  if (MODE == MODE_LIN) {
-    T xq[N], hx[N], hy[N], hz[N];
-    forward<N, T>(gtab, A, GX, GY, a, b, xq, hx, hy, hz);
-    const ColGeo<T> cg = column_geometry<N, T>(&VX[le * 32], tb, a, b, Pc, Kc);
+// Persistent form of the kernel (one wavefront per workgroup, atomic scatter): each workgroup
+// loops over element batches of a contiguous range of its XCD and loads the next batch's node
+// indices and vertices (and, for the linear and residual modes, its gathered values) into
+// registers while computing the current one.
+template <int N, typename T, int MODE, int EPB>
+__global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_persistent_kernel(
+    long nbatch, long ne, const int *__restrict__ enodes, const int *__restrict__ everts,
+    const double *__restrict__ vxyz, const double *__restrict__ kv, const T *__restrict__ ustate,
+    const T *__restrict__ xdir, T *__restrict__ y, long nowned, const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph)
+{
+  constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
+  constexpr bool PREFETCH_VALUES = MODE != MODE_JAC;
+  __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
+  __shared__ int IDX[EPB * N3];
+  __shared__ double VX[EPB * 8 * 4];
+  const int tid = threadIdx.x;
+  // contiguous batch range of this workgroup's XCD (workgroups are dealt to the 8 XCDs in turn)
+  const long per = gridDim.x / 8, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
+  const long hi = nbatch * (xcd + 1) / 8;
+  long bt = nbatch * xcd / 8 + w;
+  constexpr int NV = (EPB * 32 + NT - 1) / NT;  // vertex values (x, y, z, k of 8 vertices) per lane
+  int pidx[NS];
+  T pval[PREFETCH_VALUES ? NS : 1];
+  double pvx[NV];
+  auto load_idx = [&](long bb) {
 #pragma unroll
-    for (int k = 0; k < N; k++) {
-      T r0[3], r1[3], r2[3], det;
-      cg.eval((T)tb.phi[k], r0, r1, r2, det);
-      const T kq = ph.has_k ? (T)(Kc[0] + (Kc[1] - Kc[0]) * tb.phi[k]) : T(1);
-      // Dv = jw k grad(x) = w k (sum_m ghat_m r_m) / det;  df_m = jinv[m][:] . Dv = r_m . Dv / det
-      const T sc = wij * (T)tb.w[k] * kq * recip(det);
-      const T a0 = hx[k], a1 = hy[k], a2 = hz[k];
-      T s3[3];
-#pragma unroll
-      for (int d = 0; d < 3; d++) s3[d] = a0 * r0[d] + a1 * r1[d] + a2 * r2[d];
-      vq[k] = T(0);
-      d0[k] = sc * (r0[0] * s3[0] + r0[1] * s3[1] + r0[2] * s3[2]);
-      d1[k] = sc * (r1[0] * s3[0] + r1[1] * s3[1] + r1[2] * s3[2]);
-      d2[k] = sc * (r2[0] * s3[0] + r2[1] * s3[1] + r2[2] * s3[2]);
+    for (int s = 0; s < NS; s++) {
+      const int t = s * NT + tid, l = t / N3;
+      const long es = bb * EPB + l;
+      pidx[s] = (t < EPB * N3 && es < ne) ? enodes[es * N3 + (t - l * N3)] : -1;
     }
-  } else {
-    T xq[N], hx[N], hy[N], hz[N];
-    if constexpr (MODE == MODE_JAC) {
-      forward<N, T>(gtab, A, GX, GY, a, b, xq, hx, hy, hz);
-      __syncthreads();  // every line read before the state overwrites the tile
 #pragma unroll
-      for (int s = 0; s < N; s++) TA[s * NTH + tid] = upre[s];
+    for (int q = 0; q < NV; q++) {
+      const int t = q * NT + tid;
+      if (t < EPB * 32) {
+        const int l = t >> 5, vt = (t >> 2) & 7, d = t & 3;
+        const long es = bb * EPB + l;
+        if (es < ne) {
+          const long vi = everts[es * 8 + vt];
+          pvx[q] = d < 3 ? vxyz[3 * vi + d] : (ph.has_k ? kv[vi] : 1.0);
+        } else {  // unit cube for empty slots: finite geometry, nothing is scattered
+          pvx[q] = d == 3 ? 1.0 : (double)((vt >> (2 - d)) & 1);
+        }
+      }
+    }
+  };
+  auto load_val = [&]() {
+    if constexpr (PREFETCH_VALUES) {
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const int c = pidx[s];
+        pval[s] = MODE == MODE_FUN ? (c >= 0 ? ustate[c] : T(0)) : ((c >= 0 && c < nowned) ? xdir[c] : T(0));
+      }
+    }
+  };
+  if (bt < hi) {
+    load_idx(bt);
+    load_val();
+  }
+  for (; bt < hi; bt += per) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int t = s * NT + tid;
+      if (t < EPB * N3) {
+        IDX[t] = pidx[s];
+        if constexpr (PREFETCH_VALUES) TA[t] = pval[s];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+      if (q * NT + tid < EPB * 32) VX[q * NT + tid] = pvx[q];
+    __syncthreads();
+    T upre[MODE == MODE_JAC ? NS : 1];
+    if constexpr (!PREFETCH_VALUES) {
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const int t = s * NT + tid;
+        if (t < EPB * N3) {
+          const int c = IDX[t];
+          TA[t] = (c >= 0 && c < nowned) ? xdir[c] : T(0);
+          upre[s] = c >= 0 ? ustate[c] : T(0);
+        }
+      }
       __syncthreads();
     }
-    T uq[N], gu0[N], gu1[N], gu2[N];
-    forward<N, T>(gtab, A, GX, GY, a, b, uq, gu0, gu1, gu2);
-    const ColGeo<T> cg = column_geometry<N, T>(&VX[le * 32], tb, a, b, Pc, Kc);
-    // Pointwise physics of the state in three phases so that one copy of the transcendental
-    // code is emitted: (a) |grad u|^2 and jw per point into this thread's own line slots,
-    // (b) eta, the rank-one Jacobian factor, forcing and Bratu term in a rolled loop, (c) the
-    // per-point fluxes.  Each thread touches only its own z lines here: no barrier.
-    T *E = &A[(a * N + b) * N], *F1 = &GX[(a * N + b) * N], *U = &GY[(a * N + b) * N];
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-      T r0[3], r1[3], r2[3], det;
-      cg.eval((T)tb.phi[k], r0, r1, r2, det);
-      const T idet = recip(det);
-      const T a0 = gu0[k], a1 = gu1[k], a2 = gu2[k];
-      gu0[k] = (a0 * r0[0] + a1 * r1[0] + a2 * r2[0]) * idet;
-      gu1[k] = (a0 * r0[1] + a1 * r1[1] + a2 * r2[1]) * idet;
-      gu2[k] = (a0 * r0[2] + a1 * r1[2] + a2 * r2[2]) * idet;
-      E[k] = (T)ph.eps2 + T(0.5) * (gu0[k] * gu0[k] + gu1[k] * gu1[k] + gu2[k] * gu2[k]);
-      F1[k] = wij * (T)tb.w[k] * det;  // jw
-      U[k] = uq[k];
-    }
-    const T ex = T((ph.p - 2) / 2);
-#pragma unroll 1
-    for (int k = 0; k < N; k++) {
-      const T espg = E[k], jw = F1[k];
-      const T kq = ph.has_k ? (T)(Kc[0] + (Kc[1] - Kc[0]) * tb.phi[k]) : T(1);
-      const T eta = powe(espg, ex, ph.qpow);
-      const T lexp = ph.lambda != 0 ? T(ph.lambda) * exp(U[k]) : T(0);
-      if (MODE == MODE_FUN) {
-        double f = 0.0;
-        if (ph.exact >= 0) {
-          const double g1 = tb.phi[k], g0 = 1.0 - g1;
-          f = forcing(ph, Pc[0][0] * g0 + Pc[1][0] * g1, Pc[0][1] * g0 + Pc[1][1] * g1, Pc[0][2] * g0 + Pc[1][2] * g1);
-        }
-        E[k] = jw * eta * kq;               // Dv = jw eta k grad u
-        U[k] = -jw * (T)f + -jw * lexp;     // v = -jw f - jw lambda e^u
-      } else {
-        // Dv = jw k (eta grad x - (s . grad x) s), s = sqrt(-(p-2)/2 eta / espg) grad u; folded:
-        // Dv = E grad x - (sv . grad x) sv with sv = sqrt(jw k) s
-        E[k] = jw * eta * kq;
-        F1[k] = sqrt_pos(-ex * eta / espg * jw * kq);
-        U[k] = -jw * lexp;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-      T r0[3], r1[3], r2[3], det;
-      cg.eval((T)tb.phi[k], r0, r1, r2, det);
-      const T idet = recip(det);
-      T Dv[3];
-      if (MODE == MODE_FUN) {
-        const T sc = E[k];
-        Dv[0] = sc * gu0[k];
-        Dv[1] = sc * gu1[k];
-        Dv[2] = sc * gu2[k];
-        vq[k] = U[k];
-      } else {
-        const T b0 = hx[k], b1 = hy[k], b2 = hz[k];
-        T gxd[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) gxd[d] = (b0 * r0[d] + b1 * r1[d] + b2 * r2[d]) * idet;
-        const T c1 = F1[k], etaw = E[k];
-        const T sv0 = c1 * gu0[k], sv1 = c1 * gu1[k], sv2 = c1 * gu2[k];
-        const T dot = sv0 * gxd[0] + sv1 * gxd[1] + sv2 * gxd[2];
-        Dv[0] = etaw * gxd[0] - dot * sv0;
-        Dv[1] = etaw * gxd[1] - dot * sv1;
-        Dv[2] = etaw * gxd[2] - dot * sv2;
-        vq[k] = U[k] * xq[k];
-      }
-      d0[k] = (r0[0] * Dv[0] + r0[1] * Dv[1] + r0[2] * Dv[2]) * idet;
-      d1[k] = (r1[0] * Dv[0] + r1[1] * Dv[1] + r1[2] * Dv[2]) * idet;
-      d2[k] = (r2[0] * Dv[0] + r2[1] * Dv[1] + r2[2] * Dv[2]) * idet;
-    }
-  }
-  backward<N, T>(gtab, A, GX, GY, a, b, vq, d0, d1, d2);
-  // scatter (coalesced over the element node lists); Dirichlet rows are discarded
-#pragma unroll
-  for (int s = 0; s < N; s++) {
-    const int t = s * NTH + tid, c = IDX[t];
-    if (c >= 0 && c < nowned) {
-      const T val = TA[t];
-      if (atomic) atomicAdd(&y[c], val);
-      else y[c] += val;
-    }
+    const long nxt = bt + per;
+    if (nxt < hi) load_idx(nxt);
+    // a copy of the thread index the compiler cannot hoist: everything derived from it is
+    // recomputed per batch instead of occupying registers across the loop
+    int tid_body = tid;
+    asm volatile("" : "+v"(tid_body));
+    const int atomic = 1;
+#define DOHP_BATCH_HOOK() \
+  do {                    \
+    if (nxt < hi) load_val(); \
+  } while (0)
+#include "ellip_batch.inc"
+#undef DOHP_BATCH_HOOK
+    __syncthreads();  // the next batch overwrites the tiles
   }
 }
 
@@ -601,24 +598,15 @@ struct _p_dohpEllip {
   dohpEllipParam prm{};
   int scatter = 1;  // 0 = colours (deterministic), 1 = atomics
   bool has_state = false;
-  DevBuf<double> state, kvert, tab64;
+  int shape[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // tuned launch shape per [mode][FP32], -1 = not yet
+  float shape_ms[3][2][3] = {};
+  DevBuf<double> state, kvert, tab64, wphi;  // wphi: Gauss weights then (1 + z_q)/2
   DevBuf<float> state32, tab32;
   // host tables
   std::vector<double> B, Dh, w, phi;
 };
 
 namespace {
-
-template <int N, typename T>
-Tab<N, T> make_tab(const _p_dohpEllip *op)
-{
-  Tab<N, T> tb;
-  for (int q = 0; q < N; q++) {
-    tb.w[q] = op->w[q];
-    tb.phi[q] = op->phi[q];
-  }
-  return tb;
-}
 
 template <typename T>
 const T *gtab(const _p_dohpEllip *op);
@@ -643,34 +631,104 @@ Phys make_phys(const _p_dohpEllip *op)
   return ph;
 }
 
-template <int N, typename T, int MODE>
-dErr launch_n(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
+template <int N, typename T, int MODE, int EPB>
+void launch_shape(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st, const Phys &ph)
 {
   const dohpFS fs = op->fs;
-  constexpr int EPB = elems_per_block(N);
-  const Tab<N, T> tb = make_tab<N, T>(op);
-  const Phys ph = make_phys(op);
-  const dim3 block(EPB * N * N);
+  const dim3 block(wg_threads(N, EPB));
   if (op->scatter == 1) {
     const long ne = fs->nelem;
-    hipLaunchKernelGGL((ellip_kernel<N, T, MODE>), dim3((ne + EPB - 1) / EPB), block, 0, st, nullptr, ne,
+    hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB>), dim3((ne + EPB - 1) / EPB), block, 0, st, nullptr, ne,
                        fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr, fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y,
-                       fs->nowned, 1, gtab<T>(op), tb, ph);
+                       fs->nowned, 1, gtab<T>(op), op->wphi.ptr, ph);
   } else {
     for (int col = 0; col < 8; col++) {
       const long ne = fs->color_off[col + 1] - fs->color_off[col];
       if (!ne) continue;
-      hipLaunchKernelGGL((ellip_kernel<N, T, MODE>), dim3((ne + EPB - 1) / EPB), block, 0, st,
+      hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB>), dim3((ne + EPB - 1) / EPB), block, 0, st,
                          fs->d_color_elems.ptr + fs->color_off[col], ne, fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr,
-                         fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), tb, ph);
+                         fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), op->wphi.ptr, ph);
     }
   }
+}
+
+// Persistent launch: as many one-wavefront workgroups as fit on the device at once (a multiple of
+// the 8 XCDs), each looping over its XCD's batches.
+template <int N, typename T, int MODE, int EPB>
+dErr launch_persistent(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st, const Phys &ph)
+{
+  const dohpFS fs = op->fs;
+  const int threads = wg_threads(N, EPB);
+  int dev, per_cu = 0;
+  hipDeviceProp_t prop;
+  DOHP_HIP(hipGetDevice(&dev));
+  DOHP_HIP(hipGetDeviceProperties(&prop, dev));
+  DOHP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ellip_persistent_kernel<N, T, MODE, EPB>, threads, 0));
+  const long nbatch = (fs->nelem + EPB - 1) / EPB;
+  long grid = (long)prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
+  grid = std::min(grid, ((nbatch + 7) / 8) * 8);
+  grid = std::max(8L, grid - grid % 8);
+  hipLaunchKernelGGL((ellip_persistent_kernel<N, T, MODE, EPB>), dim3((unsigned)grid), dim3(threads), 0, st, nbatch,
+                     (long)fs->nelem, fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr, fs->d_vcoords.ptr, op->kvert.ptr, ust,
+                     x, y, fs->nowned, gtab<T>(op), op->wphi.ptr, ph);
+  return 0;
+}
+
+template <int N, typename T, int MODE>
+dErr launch_variant(const _p_dohpEllip *op, int v, const T *ust, const T *x, T *y, hipStream_t st, const Phys &ph)
+{
+  if (v == 0) launch_shape<N, T, MODE, epb_wide(N)>(op, ust, x, y, st, ph);
+  else if (v == 1) launch_shape<N, T, MODE, epb_narrow(N)>(op, ust, x, y, st, ph);
+  else DOHP_CALL((launch_persistent<N, T, MODE, epb_narrow(N)>(op, ust, x, y, st, ph)));
+  return 0;
+}
+
+// Autotuner: the first application of each (mode, precision) times the workgroup shapes (four
+// wavefronts, one wavefront, one wavefront persistent) on a scratch output and keeps the fastest
+// (the choice is part of the operator; results do not depend on it beyond the summation order
+// of the atomic scatter).
+template <int N, typename T, int MODE>
+dErr launch_n(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
+{
+  const Phys ph = make_phys(op);
+  int &shape = op->shape[MODE][sizeof(T) == 4];
+  if (op->scatter != 1 && shape == 2) shape = -1;  // the persistent form needs the atomic scatter
+  if (shape < 0) {
+    const long n = op->fs->nowned;
+    DevBuf<T> scratch;
+    DOHP_CALL(scratch.alloc(n > 0 ? n : 1));
+    hipEvent_t e0, e1;
+    DOHP_HIP(hipEventCreate(&e0));
+    DOHP_HIP(hipEventCreate(&e1));
+    float best = 0;
+    const int nvar = op->scatter == 1 ? 3 : 2;
+    for (int v = 0; v < nvar; v++) {
+      float ms = 0;
+      for (int rep = 0; rep < 3; rep++) {  // the first run of each shape is a warm-up
+        DOHP_HIP(hipEventRecord(e0, st));
+        DOHP_CALL((launch_variant<N, T, MODE>(op, v, ust, x, scratch.ptr, st, ph)));
+        DOHP_HIP(hipEventRecord(e1, st));
+        DOHP_HIP(hipEventSynchronize(e1));
+        float t;
+        DOHP_HIP(hipEventElapsedTime(&t, e0, e1));
+        if (rep > 0) ms += t;
+      }
+      op->shape_ms[MODE][sizeof(T) == 4][v] = ms / 2;
+      if (v == 0 || ms < best) {
+        best = ms;
+        shape = v;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  DOHP_CALL((launch_variant<N, T, MODE>(op, shape, ust, x, y, st, ph)));
   DOHP_HIP(hipGetLastError());
   return 0;
 }
 
 template <typename T, int MODE>
-dErr launch(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
+dErr launch(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
 {
   switch (op->fs->n) {
 #define DOHP_CASE(NN) \
@@ -763,6 +821,11 @@ extern "C" dErr dohpEllipCreate(dohpFS fs, const dohpEllipParam *prm, dohpEllip 
     if (!err) err = op->tab64.upload(t.data(), t.size());
     if (!err) err = op->tab32.upload(tf.data(), tf.size());
   }
+  {
+    std::vector<double> wp(op->w);
+    wp.insert(wp.end(), op->phi.begin(), op->phi.end());
+    if (!err) err = op->wphi.upload(wp.data(), wp.size());
+  }
   if (!err) err = op->state.alloc(fs->nclosure);
   if (!err) err = op->state32.alloc(fs->nclosure);
   if (err) {
@@ -786,6 +849,15 @@ extern "C" dErr dohpEllipSetScatter(dohpEllip op, int mode)
 {
   DOHP_CHECK(op && (mode == 0 || mode == 1), dohpERR_ARG, "scatter mode must be 0 (colours) or 1 (atomics)");
   op->scatter = mode;
+  return 0;
+}
+
+extern "C" dErr dohpEllipGetTuning(dohpEllip op, int mode, int fp32, int *shape, float *ms)
+{
+  DOHP_CHECK(op && mode >= 0 && mode < 3 && (fp32 == 0 || fp32 == 1), dohpERR_ARG, "bad argument");
+  if (shape) *shape = op->shape[mode][fp32];
+  if (ms)
+    for (int v = 0; v < 3; v++) ms[v] = op->shape_ms[mode][fp32][v];
   return 0;
 }
 

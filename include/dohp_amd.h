@@ -113,6 +113,11 @@ dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x_dev, float *y_dev, vo
 /* Scatter strategy of the element kernels: 0 = eight-colour deterministic (bitwise reproducible),
  * 1 = hardware atomics (default; summation order varies at rounding level). */
 dErr dohpEllipSetScatter(dohpEllip op, int mode);
+/* Launch shape the autotuner chose on the first application of a mode (0 = linear, 1 = residual,
+ * 2 = Jacobian action) and precision: -1 = not tuned yet, 0 = several elements per four-wavefront
+ * workgroup, 1 = one-wavefront workgroups, 2 = persistent one-wavefront workgroups with the next
+ * batch's loads in flight; ms[3] receives the measured milliseconds of each. */
+dErr dohpEllipGetTuning(dohpEllip op, int mode, int fp32, int *shape, float *ms);
 
 /* -------------------------------------------------- embedded-Q1 preconditioner */
 typedef struct _p_dohpMat *dohpMat;

@@ -27,6 +27,13 @@ selected on a held-out one, then used inside the FP32/FP64 defect-correction sol
 preconditioner taking over after as many iterations as the plain FP64 solve needed; plain FP64,
 mixed and learned-mixed solves are timed to 1e-10.
 
+--workload batch: one 32^3 hexahedral mesh (perturbed) whose elements have degrees drawn uniformly
+from 2..10 (seeded), each an independent element-local Poisson stiffness, plus 256 independent
+Poisson problems (homogeneous Dirichlet) on n^3 meshes, n uniform in 4..16, degree uniform in 2..8,
+all applied in one persistent launch (elements bucketed by degree, tile per degree autotuned).  A
+step is one FP32 application of the whole batch; FP64 is timed too.  With the CPU leg the oracle
+applies the same batch and both precisions state their error against it.
+
 --copy-probe: one plain device copy of 1 GiB (the calibration kernel for the HBM counters).
 """
 import argparse
@@ -217,6 +224,112 @@ def learned_workload(args):
     print(json.dumps(out))
 
 
+def batch_problem(Me=32, nprob=256):
+    """(mixed-mesh degrees, element vertices, [(M, degree, V)] of the independent problems)."""
+    from dohp_amd.mesh import block_vertices, element_vertices, uniform_integers
+    M = (Me, Me, Me)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=3)
+    deg = uniform_integers(4, Me ** 3, 2, 10)
+    ns, ps = uniform_integers(6, nprob, 4, 16), uniform_integers(7, nprob, 2, 8)
+    probs = []
+    for i in range(nprob):
+        Mi = (int(ns[i]),) * 3
+        probs.append((Mi, int(ps[i]), block_vertices(Mi, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1000 + i)))
+    return deg, element_vertices(M, V), probs
+
+
+def batch_workload(args):
+    import torch
+    import dohp_amd.capi as C
+    from dohp_amd.mesh import splitmix64_uniform
+    t0 = time.perf_counter()
+    Me = args.elems or 32
+    deg, vx, probs = batch_problem(Me)
+    B = C.Batch()
+    seg_mixed = B.add_elements(deg, vx)
+    segs, nbytes = [], {4: 0, 8: 0}
+    for (Mi, pi, Vi) in probs:
+        fs = C.FS(Mi, pi, Vi)
+        segs.append(B.add_space(fs))
+        for s in (4, 8):
+            nbytes[s] += alg_bytes(s, fs.nclosure, fs.nowned, pi + 1, fs.nelem, fs.nvert, False)
+        del fs
+    nexp = int(((deg + 1) ** 3).sum())
+    for s in (4, 8):  # element-local: read x, write y, 8 vertex coordinates per element
+        nbytes[s] += 2 * s * nexp + 24 * 8 * deg.size
+    B.setup()
+    n = B.size
+    x = torch.as_tensor(2 * splitmix64_uniform(9, n) - 1, dtype=torch.float64, device="cuda")
+    x32 = x.float()
+    y, y32 = torch.zeros_like(x), torch.zeros_like(x32)
+    B.mult(x, y)          # the first application of each precision tunes the tile per degree
+    B.mult_f32(x32, y32)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    nel = deg.size + sum(m[0] ** 3 for (m, _, _) in probs)
+    print(f"# setup {t_setup:.1f}s: {nel} elements, {n} packed unknowns ({nexp} element-local)",
+          file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        B.mult_f32(x32, y32, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        B.mult_f32(x32, y32, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_step = e0.elapsed_time(e1) / 1e3 / args.steps
+    t64 = time_fn(torch, lambda: B.mult(x, y, stream), max(3, args.steps // 2))
+    B.mult(x, y, stream)
+    B.mult_f32(x32, y32, stream)
+    torch.cuda.synchronize()
+    err32 = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
+    print(f"# batch fp32 {t_step * 1e6:.1f} us, fp64 {t64 * 1e6:.1f} us, fp32 vs fp64 {err32:.2e}",
+          file=sys.stderr, flush=True)
+    tuning = {}
+    for d in range(1, 11):
+        for fp32 in (0, 1):
+            tile, ms = B.tuning(d, fp32)
+            if tile >= 0:
+                tuning[f"p{d}_{'f32' if fp32 else 'f64'}"] = {"tile": ["workgroup", "wavefront"][tile], "ms": ms}
+    cpu, oracle_err = None, None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        xh, yh, y32h = x.cpu().numpy(), y.cpu().numpy(), y32.double().cpu().numpy()
+        ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        ref = np.zeros(n)
+        tc = time.perf_counter()
+        o, ln = B.segment(seg_mixed)
+        ref[o:o + ln] = O.expanded_poisson_apply(deg, vx, xh[o:o + ln], nthreads=ncores)
+        for s, (Mi, pi, Vi) in zip(segs, probs):
+            o, ln = B.segment(s)
+            S = O.BlockSpace(Mi, pi, Vi)
+            E = O.Ellip(S, p=2.0, nthreads=ncores)
+            ref[o:o + ln] = E.jacobian_apply(np.zeros(S.nclosure),
+                                             np.concatenate([xh[o:o + ln], np.zeros(S.nclosure - S.nowned)]))[:ln]
+        tc = time.perf_counter() - tc
+        nr = np.linalg.norm(ref)
+        oracle_err = {"f64": float(np.linalg.norm(yh - ref) / nr), "f32": float(np.linalg.norm(y32h - ref) / nr)}
+        cpu = {"value": round(n / tc / 1e9, 4), "unit": "GDOF/s", "cores": ncores, "kind": "port",
+               "sample": "the whole batch, one oracle application (element loop over OpenMP threads)"}
+    achieved = nbytes[4] / t_step / 1e9
+    out = {"metric": "GDOF/s batched mixed-order apply (FP32)", "value": round(n / t_step / 1e9, 3), "unit": "GDOF/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{Me}^3 mesh of degree U{{2..10}} (element-local) + {len(probs)} Poisson problems "
+                                  "n^3, n U{4..16}, degree U{2..8}, one persistent launch",
+                      "elements": int(nel), "unknowns": int(n), "element_local_unknowns": nexp},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel_us": round(t_step * 1e6, 1),
+                        "algorithmic_bytes": int(nbytes[4])},
+           "cpu_baseline": cpu,
+           "f64": {"us": round(t64 * 1e6, 1), "gdofs": round(n / t64 / 1e9, 3),
+                   "hbm_frac": round(nbytes[8] / t64 / 1e9 / HBM_PEAK_GBS, 4)},
+           "f32_rel_err_vs_f64": err32, "rel_err_vs_oracle": oracle_err, "tuning": tuning}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,7 +339,7 @@ def main():
     ap.add_argument("--degree", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scatter", type=int, default=1, help="0 = eight-colour deterministic, 1 = atomics")
-    ap.add_argument("--workload", choices=["apply", "solve", "learned"], default="apply")
+    ap.add_argument("--workload", choices=["apply", "solve", "learned", "batch"], default="apply")
     ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -239,6 +352,8 @@ def main():
         return solve_workload(args)
     if args.workload == "learned":
         return learned_workload(args)
+    if args.workload == "batch":
+        return batch_workload(args)
 
     import torch
     import dohp_amd.capi as C

@@ -438,3 +438,81 @@ class Ellip:
         if getattr(self, "h", None):
             _lib.dohpEllipDestroy(ctypes.byref(self.h))
             self.h = None
+
+
+# ------------------------------------------------ batched mixed-order Poisson operator
+_proto("dohpBatchCreate", ctypes.POINTER(_vp))
+_proto("dohpBatchDestroy", ctypes.POINTER(_vp))
+_proto("dohpBatchAddSpace", _vp, _vp, _dp, _ip)
+_proto("dohpBatchAddElements", _vp, ctypes.c_int, _ip, _dp, _dp, _ip)
+_proto("dohpBatchSetUp", _vp)
+_proto("dohpBatchGetSizes", _vp, ctypes.POINTER(_long), _ip)
+_proto("dohpBatchGetSegment", _vp, ctypes.c_int, ctypes.POINTER(_long), ctypes.POINTER(_long))
+_proto("dohpBatchMult", _vp, _vp, _vp, _vp)
+_proto("dohpBatchMultF32", _vp, _vp, _vp, _vp)
+_proto("dohpBatchGetTuning", _vp, ctypes.c_int, ctypes.c_int, _ip, _vp)
+
+
+class Batch:
+    """Linear Poisson operators of block spaces and of independent mixed-order elements, applied
+    together in one persistent launch (dohpBatch).  Vectors are packed: block spaces first, element
+    sets after them; segment(i) gives (offset, length) of segment i."""
+
+    def __init__(self):
+        self.h = _vp()
+        check(_lib.dohpBatchCreate(ctypes.byref(self.h)))
+
+    def add_space(self, fs, kvert=None):
+        seg = ctypes.c_int()
+        kp = None
+        if kvert is not None:
+            self._k = np.ascontiguousarray(kvert, dtype=np.float64)
+            kp = self._k.ctypes.data_as(_dp)
+        check(_lib.dohpBatchAddSpace(self.h, fs.h, kp, ctypes.byref(seg)))
+        return seg.value
+
+    def add_elements(self, degrees, elem_vx, kvert=None):
+        """degrees [nelem] in 1..10, elem_vx [nelem][8][3] (tensor vertex order), kvert [nelem][8]."""
+        d = np.ascontiguousarray(degrees, dtype=np.int32)
+        v = np.ascontiguousarray(elem_vx, dtype=np.float64)
+        assert v.size == 24 * d.size
+        kp = None
+        if kvert is not None:
+            k = np.ascontiguousarray(kvert, dtype=np.float64)
+            assert k.size == 8 * d.size
+            kp = k.ctypes.data_as(_dp)
+        seg = ctypes.c_int()
+        check(_lib.dohpBatchAddElements(self.h, int(d.size), d.ctypes.data_as(_ip), v.ctypes.data_as(_dp), kp,
+                                        ctypes.byref(seg)))
+        return seg.value
+
+    def setup(self):
+        check(_lib.dohpBatchSetUp(self.h))
+
+    @property
+    def size(self):
+        n, s = _long(), ctypes.c_int()
+        check(_lib.dohpBatchGetSizes(self.h, ctypes.byref(n), ctypes.byref(s)))
+        return n.value
+
+    def segment(self, i):
+        o, n = _long(), _long()
+        check(_lib.dohpBatchGetSegment(self.h, int(i), ctypes.byref(o), ctypes.byref(n)))
+        return o.value, n.value
+
+    def mult(self, x, y, stream=None):
+        check(_lib.dohpBatchMult(self.h, _ptr(x), _ptr(y), _stream(stream)))
+
+    def mult_f32(self, x, y, stream=None):
+        check(_lib.dohpBatchMultF32(self.h, _ptr(x), _ptr(y), _stream(stream)))
+
+    def tuning(self, degree, fp32=False):
+        """(tile: -1 none, 0 workgroup-packed, 1 wavefront-packed; [ms of each])."""
+        t, ms = ctypes.c_int(), (ctypes.c_float * 2)()
+        check(_lib.dohpBatchGetTuning(self.h, int(degree), int(fp32), ctypes.byref(t), ms))
+        return t.value, [round(v, 4) for v in ms]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.dohpBatchDestroy(ctypes.byref(self.h))
+            self.h = None

@@ -63,3 +63,27 @@ def dirichlet_mask_from_sets(sets) -> int:
     for s in sets:
         mask |= {100: DIRICHLET_WALLS, 200: DIRICHLET_TOP, 300: DIRICHLET_BOTTOM}[int(s)]
     return mask
+
+
+def element_vertices(M, V) -> np.ndarray:
+    """[M0*M1*M2][8][3]: vertex coordinates of each hexahedron of the block (elements in (i, j, k)
+    order, k fastest), its vertices in tensor order (a, b, c) with c fastest, the order of
+    src/fs/interface/fsgeom.c:151."""
+    M = tuple(int(m) for m in M)
+    V = np.asarray(V, dtype=np.float64).reshape(M[0] + 1, M[1] + 1, M[2] + 1, 3)
+    e0, e1, e2 = (g.reshape(-1) for g in np.meshgrid(*[np.arange(m) for m in M], indexing="ij"))
+    t = np.array([(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)])
+    return np.ascontiguousarray(V[e0[:, None] + t[None, :, 0], e1[:, None] + t[None, :, 1], e2[:, None] + t[None, :, 2]])
+
+
+def element_vertex_index(M) -> np.ndarray:
+    """[M0*M1*M2][8] block vertex index of each hexahedron's vertices in tensor order."""
+    M = tuple(int(m) for m in M)
+    e0, e1, e2 = (g.reshape(-1) for g in np.meshgrid(*[np.arange(m) for m in M], indexing="ij"))
+    t = np.array([(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)])
+    return ((e0[:, None] + t[None, :, 0]) * (M[1] + 1) + e1[:, None] + t[None, :, 1]) * (M[2] + 1) + e2[:, None] + t[None, :, 2]
+
+
+def uniform_integers(seed: int, count: int, lo: int, hi: int) -> np.ndarray:
+    """Seeded integers uniform on lo..hi (inclusive) from the splitmix64 stream."""
+    return lo + np.minimum((splitmix64_uniform(seed, count) * (hi - lo + 1)).astype(np.int64), hi - lo)

@@ -119,6 +119,43 @@ dErr dohpEllipSetScatter(dohpEllip op, int mode);
  * batch's loads in flight; ms[3] receives the measured milliseconds of each. */
 dErr dohpEllipGetTuning(dohpEllip op, int mode, int fp32, int *shape, float *ms);
 
+/* ------------------------------------------- batched mixed-order Poisson operator */
+typedef struct _p_dohpBatch *dohpBatch;
+
+/* A batch of linear Poisson operators -div(k grad u) applied together in one persistent launch
+ * (elements bucketed by order, highest first).  Two kinds of segment share one packed vector:
+ *  - block spaces: y = A x on the owned nodes of a dFSCreateBlock space, A the homogeneous-
+ *    Dirichlet stiffness (EllipShellMatMult at ellip p = 2, src/fs/tests/ellip.c:442-472);
+ *  - element sets: independent hexahedra of per-element degree 1..10 with their own n^3 GLL nodes,
+ *    y_e = K_e x_e (dEFSApply GRAD -> jw k grad -> GRAD_T per element,
+ *    src/jacobi/impls/tensor/efstopo.c:368-423, jacobi.c:383-414), the element-local form of the
+ *    mixed-order dEFS arrays of src/jacobi/tests/ex1.c:282-309 (tensor.c:417 rejects continuity
+ *    across order jumps).  Element slices follow in element order (offsets sum n_e^3,
+ *    src/fs/impls/cont/cont.c:372).
+ * Block spaces come first in the packed vector, element sets after them, each in the order added;
+ * dohpBatchGetSegment gives the offsets.  The segments' data are copied at Add time. */
+dErr dohpBatchCreate(dohpBatch *batch);
+dErr dohpBatchDestroy(dohpBatch *batch);
+/* Adds the space's operator; kvert_host: k at the space's vertices (NULL = 1). */
+dErr dohpBatchAddSpace(dohpBatch batch, dohpFS fs, const double *kvert_host, int *segment);
+/* Adds nelem hexahedra: degree_host[nelem] in 1..10, elem_vx_host [nelem][8][3] vertex coordinates
+ * in tensor order (src/fs/interface/fsgeom.c:151), kvert_host [nelem][8] or NULL (k = 1).
+ * A nonpositive Jacobian determinant at a Gauss point returns dohpERR_NEGATIVE_JACOBIAN. */
+dErr dohpBatchAddElements(dohpBatch batch, int nelem, const int *degree_host, const double *elem_vx_host,
+                          const double *kvert_host, int *segment);
+/* Freezes the batch (implicit in the calls below): packed layout, order buckets, tables. */
+dErr dohpBatchSetUp(dohpBatch batch);
+dErr dohpBatchGetSizes(dohpBatch batch, long *n, int *nsegments);
+dErr dohpBatchGetSegment(dohpBatch batch, int segment, long *offset, long *length);
+/* y = A x over every segment of the packed vectors in one persistent launch.  The first call in a
+ * precision times the two tiles of each order present (elements packed over a four-wavefront
+ * workgroup, or per wavefront) on y and keeps the faster.  FP32 keeps the geometry in FP64. */
+dErr dohpBatchMult(dohpBatch batch, const double *x_dev, double *y_dev, void *stream);
+dErr dohpBatchMultF32(dohpBatch batch, const float *x_dev, float *y_dev, void *stream);
+/* Tile chosen for a degree (-1 = none yet, 0 = workgroup-packed, 1 = wavefront-packed) and the
+ * milliseconds measured for each (-1 = not measured). */
+dErr dohpBatchGetTuning(dohpBatch batch, int degree, int fp32, int *tile, float *ms);
+
 /* -------------------------------------------------- embedded-Q1 preconditioner */
 typedef struct _p_dohpMat *dohpMat;
 

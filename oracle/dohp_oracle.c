@@ -900,3 +900,49 @@ int orc_ellip_error_norms(const orc_problem *pb, const double *uc, double en[3],
   rule_free(&r);
   return err;
 }
+
+/* Element-local stiffness of independent hexahedra of per-element degree: for each element, the
+ * Jacobian action of ellip at p = 2 (EllipShellMatMult, ellip.c:442-472: GRAD, Dv = jw k grad,
+ * GRAD_T) on a one-element space with no Dirichlet nodes, i.e. K_e x_e.  Element slices are
+ * consecutive, offsets sum (degree+1)^3 in element order (cont.c:372); elements of different
+ * degree get their own rules and bases, as the mixed-order dEFS arrays of jacobi/tests/ex1.c:282-309. */
+int orc_expanded_poisson_apply(int nelem, const int *degree, const double *elem_vx, const double *kvert,
+                               const double *x, double *y, int nthreads)
+{
+  long *off = malloc(sizeof(long) * (nelem + 1));
+  off[0] = 0;
+  for (int e = 0; e < nelem; e++) {
+    const long n = degree[e] + 1;
+    off[e + 1] = off[e] + n * n * n;
+  }
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 8) reduction(| : err)
+  for (int e = 0; e < nelem; e++) {
+    const int n = degree[e] + 1, np = n * n * n;
+    int *id = malloc(sizeof(int) * np);
+    double *zero = calloc(np, sizeof(double));
+    for (int i = 0; i < np; i++) id[i] = i;
+    orc_problem pb;
+    memset(&pb, 0, sizeof(pb));
+    pb.nelem = 1;
+    pb.p = degree[e];
+    pb.nclosure = pb.nowned = np;
+    pb.elem_nodes = id;
+    pb.elem_vx = &elem_vx[(long)e * 24];
+    pb.ep = 2.0;
+    pb.eps = 1.0;
+    pb.fmethod = ORC_QUAD_FAST;
+    pb.nthreads = 1;
+    pb.kvert = kvert ? &kvert[(long)e * 8] : NULL;
+    err |= ellip_apply(&pb, 1, zero, &x[off[e]], &y[off[e]]);
+    free(id);
+    free(zero);
+  }
+  free(off);
+  return err;
+}

@@ -75,6 +75,11 @@ long orc_ellip_q1_assemble(const orc_problem *pb, const double *uc, int qmethod,
 void orc_set_q1_cjinv_patch_order(int flag);
 /* Error norms (util.c:110-168 via ellip.c:538-564). */
 int orc_ellip_error_norms(const orc_problem *pb, const double *uc, double enorm[3], double gnorm[3]);
+/* Element-local Poisson stiffness K_e x_e of independent elements of per-element degree
+ * (ellip.c:442-472 at p = 2 on one-element spaces; slices at offsets sum (degree+1)^3).
+ * elem_vx [nelem][8][3] tensor order, kvert [nelem][8] or NULL. */
+int orc_expanded_poisson_apply(int nelem, const int *degree, const double *elem_vx, const double *kvert,
+                               const double *x, double *y, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -251,3 +251,19 @@ class Ellip:
         en, gn = np.zeros(3), np.zeros(3)
         assert lib().orc_ellip_error_norms(ctypes.byref(self.pb), _p(uc), _p(en), _p(gn)) == 0
         return en, gn
+
+
+def expanded_poisson_apply(degrees, elem_vx, x, kvert=None, nthreads=0):
+    """K_e x_e for independent hexahedra of per-element degree (element slices consecutive,
+    offsets sum (degree+1)^3): the Jacobian action of ellip at p = 2 (ellip.c:442-472) on
+    one-element spaces without Dirichlet nodes.  elem_vx [nelem][8][3], kvert [nelem][8]."""
+    d = np.ascontiguousarray(degrees, dtype=np.int32)
+    v = np.ascontiguousarray(elem_vx, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    assert v.size == 24 * d.size and xx.size == int(((d + 1) ** 3).sum())
+    k = None if kvert is None else np.ascontiguousarray(kvert, dtype=np.float64)
+    y = np.zeros_like(xx)
+    rc = lib().orc_expanded_poisson_apply(int(d.size), _p(d, _ip), _p(v), _p(k) if k is not None else None,
+                                          _p(xx), _p(y), int(nthreads))
+    assert rc == 0, rc
+    return y

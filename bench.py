@@ -287,12 +287,36 @@ def batch_workload(args):
     err32 = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
     print(f"# batch fp32 {t_step * 1e6:.1f} us, fp64 {t64 * 1e6:.1f} us, fp32 vs fp64 {err32:.2e}",
           file=sys.stderr, flush=True)
+    names = ["workgroup", "wavefront", "mfma"]
     tuning = {}
     for d in range(1, 11):
         for fp32 in (0, 1):
             tile, ms = B.tuning(d, fp32)
             if tile >= 0:
-                tuning[f"p{d}_{'f32' if fp32 else 'f64'}"] = {"tile": ["workgroup", "wavefront"][tile], "ms": ms}
+                tuning[f"p{d}_{'f32' if fp32 else 'f64'}"] = {"tile": names[tile], "ms": ms}
+    # FP32 with the MFMA contraction tile offered to the tuner from degree 4 up (BF16 operands, FP32
+    # accumulation; 3 = operands split into two BF16 parts), and with it fixed for every degree >= 4
+    mfma, ymf = {}, {}
+    for passes in (3, 1):
+        for forced in (False, True):
+            B.set_mfma(passes)
+            for d in range(4, 11):
+                B.set_tile(d, True, 2 if forced else -1)
+            ym = torch.zeros_like(y32)
+            B.mult_f32(x32, ym, stream)
+            tm = time_fn(torch, lambda: B.mult_f32(x32, ym, stream), max(3, args.steps // 2))
+            B.mult_f32(x32, ym, stream)
+            torch.cuda.synchronize()
+            key = f"bf16x{passes}_{'all' if forced else 'tuned'}"
+            ymf[key] = ym.double().cpu().numpy()
+            mfma[key] = {"us": round(tm * 1e6, 1), "gdofs": round(n / tm / 1e9, 3),
+                         "rel_err_vs_f64": float(torch.linalg.norm(ym.double() - y) / torch.linalg.norm(y)),
+                         "tiles": {f"p{d}": names[B.tuning(d, True)[0]] for d in range(4, 11)
+                                   if B.tuning(d, True)[0] >= 0}}
+            print(f"# batch fp32 {key}: {mfma[key]}", file=sys.stderr, flush=True)
+    B.set_mfma(0)
+    for d in range(4, 11):
+        B.set_tile(d, True, -1)
     cpu, oracle_err = None, None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -311,6 +335,8 @@ def batch_workload(args):
         tc = time.perf_counter() - tc
         nr = np.linalg.norm(ref)
         oracle_err = {"f64": float(np.linalg.norm(yh - ref) / nr), "f32": float(np.linalg.norm(y32h - ref) / nr)}
+        for key, ym in ymf.items():
+            oracle_err[f"f32_{key}"] = float(np.linalg.norm(ym - ref) / nr)
         cpu = {"value": round(n / tc / 1e9, 4), "unit": "GDOF/s", "cores": ncores, "kind": "port",
                "sample": "the whole batch, one oracle application (element loop over OpenMP threads)"}
     achieved = nbytes[4] / t_step / 1e9
@@ -326,7 +352,7 @@ def batch_workload(args):
            "cpu_baseline": cpu,
            "f64": {"us": round(t64 * 1e6, 1), "gdofs": round(n / t64 / 1e9, 3),
                    "hbm_frac": round(nbytes[8] / t64 / 1e9 / HBM_PEAK_GBS, 4)},
-           "f32_rel_err_vs_f64": err32, "rel_err_vs_oracle": oracle_err, "tuning": tuning}
+           "f32_rel_err_vs_f64": err32, "rel_err_vs_oracle": oracle_err, "tuning": tuning, "f32_mfma": mfma}
     print(json.dumps(out))
 
 

@@ -451,6 +451,8 @@ _proto("dohpBatchGetSegment", _vp, ctypes.c_int, ctypes.POINTER(_long), ctypes.P
 _proto("dohpBatchMult", _vp, _vp, _vp, _vp)
 _proto("dohpBatchMultF32", _vp, _vp, _vp, _vp)
 _proto("dohpBatchGetTuning", _vp, ctypes.c_int, ctypes.c_int, _ip, _vp)
+_proto("dohpBatchSetMFMA", _vp, ctypes.c_int)
+_proto("dohpBatchSetTile", _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int)
 
 
 class Batch:
@@ -506,9 +508,17 @@ class Batch:
     def mult_f32(self, x, y, stream=None):
         check(_lib.dohpBatchMultF32(self.h, _ptr(x), _ptr(y), _stream(stream)))
 
+    def set_mfma(self, passes):
+        """FP32 MFMA contraction tiles from degree 4 up: 0 off, 1 BF16, 3 split BF16 (three MFMAs)."""
+        check(_lib.dohpBatchSetMFMA(self.h, int(passes)))
+
+    def set_tile(self, degree, fp32, tile):
+        """Fix the tile of a degree and precision (-1 = tune)."""
+        check(_lib.dohpBatchSetTile(self.h, int(degree), int(fp32), int(tile)))
+
     def tuning(self, degree, fp32=False):
-        """(tile: -1 none, 0 workgroup-packed, 1 wavefront-packed; [ms of each])."""
-        t, ms = ctypes.c_int(), (ctypes.c_float * 2)()
+        """(tile: -1 none, 0 workgroup-packed, 1 wavefront-packed, 2 MFMA; [ms of each])."""
+        t, ms = ctypes.c_int(), (ctypes.c_float * 3)()
         check(_lib.dohpBatchGetTuning(self.h, int(degree), int(fp32), ctypes.byref(t), ms))
         return t.value, [round(v, 4) for v in ms]
 

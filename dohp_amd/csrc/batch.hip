@@ -18,7 +18,8 @@
 // kernel through an atomic counter, highest order first.  The tile of each order (elements packed
 // over the four wavefronts of a workgroup with workgroup barriers, or packed per wavefront with
 // wavefront-local synchronization) is chosen by timing both on the first application in each
-// precision.
+// precision.  From degree 4 up, FP32 applies may also offer the tuner a third tile whose contractions are
+// wavefront-level GEMMs on the matrix cores (BF16 operands, FP32 accumulation; dohpBatchSetMFMA).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -33,7 +34,8 @@ namespace {
 
 constexpr int kBT = 256;                  // threads of a batch workgroup
 constexpr int kNMin = 2, kNMax = 11;      // nodes per direction (degree 1..10)
-enum { PACK_BLOCK = 0, PACK_WAVE = 1 };
+enum { PACK_BLOCK = 0, PACK_WAVE = 1, PACK_MFMA = 2 };
+constexpr int kPacks = 3;
 
 // Offsets of an order's tables in the concatenated tables of n = 2..11.
 constexpr int tab_off(int N)
@@ -43,10 +45,22 @@ constexpr int tab_off(int N)
   return s;
 }
 constexpr int wphi_off(int N) { return N * (N - 1) - 2; }  // sum of 2m over m = 2..N-1
+constexpr int ftab_off(int N)                               // full B, Dh, B^T, Dh^T per order
+{
+  int s = 0;
+  for (int m = kNMin; m < N; m++) s += 4 * m * m;
+  return s;
+}
 constexpr int epw(int N) { return 64 / (N * N); }           // elements per wavefront (0 if n^2 > 64)
-constexpr int item_elems(int N, int pack) { return pack == PACK_BLOCK ? kBT / (N * N) : 4 * epw(N); }
-constexpr int cfg_id(int N, int pack) { return (N - kNMin) * 2 + pack; }
-inline bool pack_valid(int N, int pack) { return pack == PACK_BLOCK || epw(N) > 0; }
+constexpr int item_elems(int N, int pack) { return pack == PACK_WAVE ? 4 * epw(N) : kBT / (N * N); }
+constexpr int cfg_id(int N, int pack) { return (N - kNMin) * kPacks + pack; }
+constexpr int kMfmaMinN = 5;  // MFMA contractions from degree 4 up
+inline bool pack_valid(int N, int pack, int tsize, int mfma)
+{
+  if (pack == PACK_WAVE) return epw(N) > 0;
+  if (pack == PACK_MFMA) return mfma > 0 && tsize == 4 && N >= kMfmaMinN;
+  return true;
+}
 inline size_t cfg_lds(int N, int pack, int tsize)
 {
   const size_t E = item_elems(N, pack), N3 = (size_t)N * N * N;
@@ -62,8 +76,73 @@ struct BArgs {
   const double *vxyz;  // [vertices][3]
   const double *kv;    // [vertices] coefficient k
   const double *wphi;  // per order: Gauss weights then (1 + z_q)/2
+  const float *ftab;   // per order: B, Dh, B^T, Dh^T as full n x n matrices [q][m]
+  int passes;          // MFMA tiles: 1 = BF16 operands, 3 = operands split into two BF16 parts
   long goff[kNMax + 1], eoff[kNMax + 1];
 };
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short bf16_bits(float v) { return __builtin_bit_cast(short, (__bf16)v); }
+__device__ __forceinline__ float bf16_value(short h) { return __builtin_bit_cast(float, (unsigned)(unsigned short)h << 16); }
+
+// A 1-D operator Op[q][m] (n x n) as the B operand of v_mfma_f32_16x16x16_bf16: lane l holds
+// Op[q = l % 16][m = 4 (l / 16) + i], i = 0..3, zero outside n x n; hi is its BF16 rounding, lo the BF16
+// rounding of the rest.
+template <int N>
+__device__ __forceinline__ void mfma_operator(const float *__restrict__ op, int lane, bf16x4 &hi, bf16x4 &lo)
+{
+  const int q = lane & 15, kg = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int m = 4 * kg + i;
+    const float v = (q < N && m < N) ? op[q * N + m] : 0.f;
+    hi[i] = bf16_bits(v);
+    lo[i] = bf16_bits(v - bf16_value(hi[i]));
+  }
+}
+
+// One contraction along DIR of the E element tiles of a workgroup as wavefront-level GEMMs: the lines of a tile
+// (n^2 of them) are the GEMM rows, taken 16 at a time; row block x operator is one 16x16x16 BF16 MFMA with FP32
+// accumulation (three with split operands: hi.hi + lo.hi + hi.lo).  Wavefront w takes the (element, row block)
+// pairs w, w + 4, ...  out = Op in (ACC: out += Op in); in == out is allowed, since a block's rows are read
+// before they are written and blocks share no lines.
+template <int N, int DIR, bool ACC, int E>
+__device__ __forceinline__ void mfma_contract(const float *in, float *out, bf16x4 oh, bf16x4 ol, int passes,
+                                              int wave, int lane)
+{
+  constexpr int NN = N * N, N3 = NN * N, NB = (NN + 15) / 16;
+  const int r16 = lane & 15, kg = lane >> 4;
+  for (int p = wave; p < E * NB; p += 4) {
+    const int e = p / NB, rb = p - e * NB;
+    const float *ti = in + e * N3;
+    float *to = out + e * N3;
+    const int r = rb * 16 + r16, ra = r / N, rr = r - ra * N;
+    bf16x4 xh, xl;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int m = 4 * kg + i;
+      const float v = (r < NN && m < N) ? ti[line_at<N, DIR>(ra, rr, m)] : 0.f;
+      xh[i] = bf16_bits(v);
+      xl[i] = bf16_bits(v - bf16_value(xh[i]));
+    }
+    floatx4 c = {0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xh, oh, c, 0, 0, 0);
+    if (passes > 1) {
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xl, oh, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xh, ol, c, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r2 = rb * 16 + 4 * kg + i, a2 = r2 / N, b2 = r2 - a2 * N;
+      if (r2 < NN && r16 < N) {
+        float *o = &to[line_at<N, DIR>(a2, b2, r16)];
+        *o = ACC ? *o + c[i] : c[i];
+      }
+    }
+  }
+}
 
 // One item: up to item_elems(N, PACK) consecutive elements of the order-N bucket.  Loader groups
 // are the whole workgroup (PACK_BLOCK) or each wavefront for its own elements (PACK_WAVE).
@@ -134,7 +213,31 @@ __device__ __forceinline__ void batch_item(const BArgs &g, const int4 it, const 
   const T wij = (T)(wl[a] * wl[b]);
 
   T xq[N], hx[N], hy[N], hz[N];
-  forward<N, T, WS>(tab + tab_off(N), A, GX, GY, a, b, xq, hx, hy, hz);
+  const int wave = tid >> 6, lane = tid & 63;
+  const float *fop = g.ftab + ftab_off(N);
+  if constexpr (PACK == PACK_MFMA) {
+    // interpolation along z, y, x, then the collocated derivatives along x and y into GX and GY, and along z in
+    // place (the linear operator needs no values at the points)
+    bf16x4 oh, ol;
+    mfma_operator<N>(fop, lane, oh, ol);
+    mfma_contract<N, DIR_Z, false, E>(TA, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_contract<N, DIR_Y, false, E>(TA, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_contract<N, DIR_X, false, E>(TA, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_operator<N>(fop + NN, lane, oh, ol);
+    mfma_contract<N, DIR_X, false, E>(TA, TGX, oh, ol, g.passes, wave, lane);
+    mfma_contract<N, DIR_Y, false, E>(TA, TGY, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_contract<N, DIR_Z, false, E>(TA, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    ld_line<N, DIR_Z>(GX, a, b, hx);
+    ld_line<N, DIR_Z>(GY, a, b, hy);
+    ld_line<N, DIR_Z>(A, a, b, hz);
+  } else {
+    forward<N, T, WS>(tab + tab_off(N), A, GX, GY, a, b, xq, hx, hy, hz);
+  }
   double Pc[2][3], Kc[2];
   const ColGeo<T> cg = column_geometry<N, T>(&VX[le * 32], wl + N, a, b, Pc, Kc);
   T vq[N], d0[N], d1[N], d2[N];
@@ -154,7 +257,32 @@ __device__ __forceinline__ void batch_item(const BArgs &g, const int4 it, const 
     d1[k] = sc * (r1[0] * s3[0] + r1[1] * s3[1] + r1[2] * s3[2]);
     d2[k] = sc * (r2[0] * s3[0] + r2[1] * s3[1] + r2[2] * s3[2]);
   }
-  backward<N, T, WS>(tab + tab_off(N), A, GX, GY, a, b, vq, d0, d1, d2);
+  if constexpr (PACK == PACK_MFMA) {
+    // fluxes back into the tiles (each thread its own z line), then Dh^T along z in place, x and y added, and
+    // B^T along x, y, z
+    st_line<N, DIR_Z>(GX, a, b, d0);
+    st_line<N, DIR_Z>(GY, a, b, d1);
+    st_line<N, DIR_Z>(A, a, b, d2);
+    __syncthreads();
+    bf16x4 oh, ol;
+    mfma_operator<N>(fop + 3 * NN, lane, oh, ol);
+    mfma_contract<N, DIR_Z, false, E>(TA, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_contract<N, DIR_X, true, E>(TGX, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_contract<N, DIR_Y, true, E>(TGY, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_operator<N>(fop + 2 * NN, lane, oh, ol);
+    mfma_contract<N, DIR_X, false, E>(TA, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_contract<N, DIR_Y, false, E>(TA, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    mfma_contract<N, DIR_Z, false, E>(TA, TA, oh, ol, g.passes, wave, lane);
+    __syncthreads();
+    (void)vq;
+  } else {
+    backward<N, T, WS>(tab + tab_off(N), A, GX, GY, a, b, vq, d0, d1, d2);
+  }
 
   // scatter: atomics into block spaces, plain stores into element-local slices
   int tl = gt;
@@ -193,6 +321,10 @@ __global__ void __launch_bounds__(kBT) batch_kernel(BArgs g, const T *__restrict
 #define DOHP_BCASE2(NN_)                                                                         \
   DOHP_BCASE(NN_)                                                                                \
   case cfg_id(NN_, PACK_WAVE): batch_item<NN_, T, PACK_WAVE>(g, it, tab, x, y, smem); break;
+#define DOHP_BCASEM(NN_)                                                                         \
+  case cfg_id(NN_, PACK_MFMA):                                                                   \
+    if constexpr (sizeof(T) == 4) batch_item<NN_, T, PACK_MFMA>(g, it, tab, x, y, smem);        \
+    break;
       DOHP_BCASE2(2)
       DOHP_BCASE2(3)
       DOHP_BCASE2(4)
@@ -203,6 +335,14 @@ __global__ void __launch_bounds__(kBT) batch_kernel(BArgs g, const T *__restrict
       DOHP_BCASE(9)
       DOHP_BCASE(10)
       DOHP_BCASE(11)
+      DOHP_BCASEM(5)
+      DOHP_BCASEM(6)
+      DOHP_BCASEM(7)
+      DOHP_BCASEM(8)
+      DOHP_BCASEM(9)
+      DOHP_BCASEM(10)
+      DOHP_BCASEM(11)
+#undef DOHP_BCASEM
 #undef DOHP_BCASE2
 #undef DOHP_BCASE
       default: break;
@@ -251,7 +391,10 @@ struct _p_dohpBatch {
   DevBuf<float> tab32;
   // per precision: tile per order (-1 = not tuned), measured ms per tile, item list
   int pack[2][kNMax + 1];
-  float pack_ms[2][kNMax + 1][2];
+  float pack_ms[2][kNMax + 1][kPacks];
+  int mfma = 0;  // 0 = VALU contractions only; 1 or 3 = MFMA tiles (BF16 passes) offered to the FP32 tuner
+  int forced[2][kNMax + 1];  // tile fixed by the caller, -1 = tuned
+  DevBuf<float> ftab;
   DevBuf<int4> items[2];
   int nitems[2] = {0, 0};
   size_t lds[2] = {0, 0};
@@ -259,8 +402,8 @@ struct _p_dohpBatch {
   {
     for (int p = 0; p < 2; p++)
       for (int N = 0; N <= kNMax; N++) {
-        pack[p][N] = -1;
-        pack_ms[p][N][0] = pack_ms[p][N][1] = -1.f;
+        pack[p][N] = forced[p][N] = -1;
+        for (int k = 0; k < kPacks; k++) pack_ms[p][N][k] = -1.f;
       }
   }
 };
@@ -357,7 +500,25 @@ dErr batch_setup(_p_dohpBatch *b)
       wp[wphi_off(N) + N + q] = 0.5 * (1.0 + z[q]);
     }
   }
-  std::vector<float> t32(t64.begin(), t64.end());
+  std::vector<float> t32(t64.begin(), t64.end()), tf(ftab_off(kNMax + 1), 0.f);
+  for (int N = kNMin; N <= kNMax; N++) {
+    double z[kMaxN], w[kMaxN], zl[kMaxN], wl[kMaxN];
+    gauss_legendre(N, z, w);
+    gauss_lobatto(N, zl, wl);
+    std::vector<double> B(N * N), Dh(N * N);
+    lagrange_matrix(N, zl, N, z, B.data());
+    collocation_derivative(N, z, Dh.data());
+    float *f = &tf[ftab_off(N)];
+    for (int q = 0; q < N; q++)
+      for (int m = 0; m < N; m++) {
+        f[q * N + m] = (float)B[q * N + m];
+        f[N * N + q * N + m] = (float)Dh[q * N + m];
+        f[2 * N * N + q * N + m] = (float)B[m * N + q];
+        f[3 * N * N + q * N + m] = (float)Dh[m * N + q];
+      }
+  }
+  DOHP_CALL(b->ftab.upload(tf.data(), tf.size()));
+  b->args.ftab = b->ftab.ptr;
   DOHP_CALL(b->gidx.upload(gidx.data(), gidx.size()));
   DOHP_CALL(b->everts.upload(ev.data(), ev.size()));
   DOHP_CALL(b->vxyz.upload(vx.data(), vx.size()));
@@ -416,6 +577,7 @@ dErr batch_launch(_p_dohpBatch *b, const int4 *items, int nitems, size_t lds, co
   BArgs a = b->args;
   a.items = items;
   a.nitems = nitems;
+  a.passes = b->mfma > 1 ? 3 : 1;
   hipLaunchKernelGGL(batch_kernel<T>, dim3(grid), dim3(kBT), lds, st, a, tabs<T>(b), x, y);
   DOHP_HIP(hipGetLastError());
   return 0;
@@ -435,8 +597,14 @@ dErr batch_tune(_p_dohpBatch *b, const T *x, T *y, hipStream_t st)
   for (int N = kNMin; N <= kNMax && !err; N++) {
     if (b->bucket_n[N] == 0) continue;
     float best = 1e30f;
-    for (int p = 0; p < 2 && !err; p++) {
-      if (!pack_valid(N, p)) continue;
+    if (b->forced[pr][N] >= 0) {
+      DOHP_CHECK(pack_valid(N, b->forced[pr][N], sizeof(T), b->mfma), dohpERR_ARG,
+                 "tile not available for degree " + std::to_string(N - 1));
+      b->pack[pr][N] = b->forced[pr][N];
+      continue;
+    }
+    for (int p = 0; p < kPacks && !err; p++) {
+      if (!pack_valid(N, p, sizeof(T), b->mfma)) continue;
       int one[kNMax + 1];
       for (int m = 0; m <= kNMax; m++) one[m] = m == N ? p : -1;
       const std::vector<int4> it = build_items(b, one);
@@ -607,14 +775,39 @@ extern "C" dErr dohpBatchMultF32(dohpBatch b, const float *x, float *y, void *st
   return batch_mult<float>(b, x, y, stream);
 }
 
+extern "C" dErr dohpBatchSetMFMA(dohpBatch b, int passes)
+{
+  DOHP_CHECK(b && (passes == 0 || passes == 1 || passes == 3), dohpERR_ARG, "MFMA passes must be 0, 1 or 3");
+  if (b->mfma != passes) {  // the FP32 tiles are tuned again on the next application
+    b->mfma = passes;
+    b->items[1].release();
+    b->nitems[1] = 0;
+    for (int N = 0; N <= kNMax; N++) {
+      b->pack[1][N] = -1;
+      for (int k = 0; k < kPacks; k++) b->pack_ms[1][N][k] = -1.f;
+    }
+  }
+  return 0;
+}
+
+extern "C" dErr dohpBatchSetTile(dohpBatch b, int degree, int fp32, int tile)
+{
+  DOHP_CHECK(b && degree >= 1 && degree <= 10 && (fp32 == 0 || fp32 == 1) && tile >= -1 && tile < kPacks,
+             dohpERR_ARG, "bad argument");
+  b->forced[fp32][degree + 1] = tile;
+  b->items[fp32].release();
+  b->nitems[fp32] = 0;
+  b->pack[fp32][degree + 1] = -1;
+  return 0;
+}
+
 extern "C" dErr dohpBatchGetTuning(dohpBatch b, int degree, int fp32, int *tile, float *ms)
 {
   DOHP_CHECK(b && degree >= 1 && degree <= 10 && (fp32 == 0 || fp32 == 1), dohpERR_ARG, "bad argument");
   const int N = degree + 1;
   if (tile) *tile = b->pack[fp32][N];
   if (ms) {
-    ms[0] = b->pack_ms[fp32][N][0];
-    ms[1] = b->pack_ms[fp32][N][1];
+    for (int k = 0; k < kPacks; k++) ms[k] = b->pack_ms[fp32][N][k];
   }
   return 0;
 }

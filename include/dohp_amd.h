@@ -152,8 +152,16 @@ dErr dohpBatchGetSegment(dohpBatch batch, int segment, long *offset, long *lengt
  * workgroup, or per wavefront) on y and keeps the faster.  FP32 keeps the geometry in FP64. */
 dErr dohpBatchMult(dohpBatch batch, const double *x_dev, double *y_dev, void *stream);
 dErr dohpBatchMultF32(dohpBatch batch, const float *x_dev, float *y_dev, void *stream);
-/* Tile chosen for a degree (-1 = none yet, 0 = workgroup-packed, 1 = wavefront-packed) and the
- * milliseconds measured for each (-1 = not measured). */
+/* MFMA contractions for FP32 applies from degree 4 up (no reference counterpart): passes = 0 keeps the
+ * FP32 VALU contractions (default); 1 offers the tuner a tile whose contractions are 16x16x16 BF16
+ * MFMAs with FP32 accumulation (relative error ~1e-3); 3 splits both operands into two BF16 parts and
+ * sums three MFMAs per tile (~1e-5).  Changing it retunes the FP32 tiles on the next application. */
+dErr dohpBatchSetMFMA(dohpBatch batch, int passes);
+/* Fixes the tile of a degree and precision instead of timing it (-1 = tune again); an unavailable tile
+ * is an error at the next application. */
+dErr dohpBatchSetTile(dohpBatch batch, int degree, int fp32, int tile);
+/* Tile chosen for a degree (-1 = none yet, 0 = workgroup-packed, 1 = wavefront-packed, 2 = MFMA) and
+ * the milliseconds measured for each, ms[3] (-1 = not measured). */
 dErr dohpBatchGetTuning(dohpBatch batch, int degree, int fp32, int *tile, float *ms);
 
 /* -------------------------------------------------- embedded-Q1 preconditioner */

@@ -101,7 +101,7 @@ def test_batch_fp64_and_fp32_match_oracle(C):
     for d in present:
         for fp32 in (False, True):
             tile, ms = B.tuning(d, fp32)
-            assert tile in (0, 1) and ms[tile] > 0
+            assert tile in (0, 1) and ms[tile] > 0 and ms[2] < 0
 
 
 def test_batch_repeat_and_layout(C):
@@ -148,3 +148,35 @@ def test_batch_rejects_bad_input(C):
     with pytest.raises(C.DohpError) as ei:
         B.add_elements([3], mirrored)
     assert ei.value.code == C.ERR_NEGATIVE_JACOBIAN
+
+
+@pytest.mark.parametrize("passes,tol", [(3, 5e-4), (1, 5e-2)])
+def test_batch_mfma_tiles(C, passes, tol):
+    """FP32 with the MFMA contraction tile (BF16 operands, FP32 accumulation) fixed for every degree >= 4,
+    against the FP64 oracle per segment; then the tuner with the MFMA tile offered measures all three."""
+    print(f"batch mfma passes={passes}", flush=True)
+    spaces, elems = case(45)
+    B, keep, segs = build(C, spaces, elems)
+    B.set_mfma(passes)
+    for d in range(4, 11):
+        B.set_tile(d, True, 2)
+    n = B.size
+    x = 2 * splitmix64_uniform(8, n) - 1
+    y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    B.mult_f32(dev(x, torch.float32), y32)
+    y32h = y32.cpu().numpy()
+    errs = []
+    for kind, s, data in segs:
+        off, ln = B.segment(s)
+        ref = reference(kind, data, x[off:off + ln])
+        errs.append(rel(y32h[off:off + ln], ref))
+        assert errs[-1] < tol, (kind, s, errs[-1])
+    print("mfma tile errors", ["%.2e" % e for e in errs])
+    for d in range(4, 11):
+        B.set_tile(d, True, -1)
+    B.mult_f32(dev(x, torch.float32), y32)
+    for d in range(4, 11):
+        tile, ms = B.tuning(d, True)
+        if tile >= 0:  # workgroup and MFMA tiles measured (the wavefront tile only for degree <= 7)
+            assert ms[0] > 0 and ms[2] > 0 and (ms[1] > 0) == (d <= 7), (d, ms)
+    print("tiles", {d: B.tuning(d, True) for d in range(1, 11)})

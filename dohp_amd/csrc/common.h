@@ -138,7 +138,8 @@ struct _p_dohpFS {
   std::vector<int> color_off;        // 9 offsets into color_elems
   std::vector<int> color_elems;      // elements grouped by (ex%2, ey%2, ez%2)
   std::vector<int> trav_elems;       // all elements in Morton (Z-curve) order
-  dohp::DevBuf<int> d_elem_nodes, d_elem_verts, d_color_elems, d_trav_elems;
+  std::vector<int> lat2clo;          // closure index of each lattice node (I, J, K), K fastest
+  dohp::DevBuf<int> d_elem_nodes, d_elem_verts, d_color_elems, d_trav_elems, d_lat2clo;
   dohp::DevBuf<double> d_vcoords, d_dcache;
   dohp::DevBuf<float> d_dcache32;
 };

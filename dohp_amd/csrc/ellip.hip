@@ -21,11 +21,45 @@ using namespace dohp;
 
 namespace {
 
+// Node and vertex numbering of a block space, evaluated instead of read from the element maps:
+// element e = (ex M1 + ey) M2 + ez has node (a, b, c) at lattice point (ex p + a, ey p + b, ez p + c)
+// and vertex (ta, tb, tc) at ((ex + ta) V1 + ey + tb) V2 + ez + tc; the owned nodes are the box
+// [I0, I0 + nx) x [J0, J0 + ny) x [K0, K0 + nz) of the lattice in lexicographic order (the closure
+// numbering of dFSCreateBlock), and the state is kept in lattice order (Dirichlet values included).
+struct BoxMap {
+  int p, M1, M2, V1, V2, Ly, Lz, I0, J0, K0, nx, ny, nz;
+};
 
+__device__ __forceinline__ void elem_coords(const BoxMap &m, int e, int &ex, int &ey, int &ez)
+{
+  const int q = e / m.M2;
+  ez = e - q * m.M2;
+  ex = q / m.M1;
+  ey = q - ex * m.M1;
+}
+__device__ __forceinline__ int owned_index(const BoxMap &m, int I, int J, int K)
+{
+  const int i = I - m.I0, j = J - m.J0, k = K - m.K0;
+  return ((unsigned)i < (unsigned)m.nx && (unsigned)j < (unsigned)m.ny && (unsigned)k < (unsigned)m.nz)
+             ? (i * m.ny + j) * m.nz + k
+             : -1;
+}
+__device__ __forceinline__ long vertex_index(const BoxMap &m, int ex, int ey, int ez, int vt)
+{
+  return ((long)(ex + (vt >> 2)) * m.V1 + ey + ((vt >> 1) & 1)) * m.V2 + ez + (vt & 1);
+}
+// node r = (a n + b) n + c of an element whose first lattice point is (I, J, K)
+template <int N>
+__device__ __forceinline__ void node_lattice(int r, int &I, int &J, int &K)
+{
+  const int a = r / (N * N), r2 = r - a * N * N, b = r2 / N;
+  I += a;
+  J += b;
+  K += r2 - b * N;
+}
 
 template <int N, typename T, int MODE, int EPB>
-__global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__restrict__ elist, long ne,
-                                                    const int *__restrict__ enodes, const int *__restrict__ everts,
+__global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__restrict__ elist, long ne, BoxMap bm,
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
                                                     const T *__restrict__ ustate, const T *__restrict__ xdir,
                                                     T *__restrict__ y, long nowned, int atomic,
@@ -35,55 +69,58 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__
   __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
   __shared__ int IDX[EPB * N3];
   __shared__ double VX[EPB * 8 * 4];
+  __shared__ int EB[EPB * 4];  // per element slot: first lattice point, 1 if the slot holds an element
 
   const long lb = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
-  const int line = tid % (EPB * NN), le = line / NN, ab = line - le * NN, a = ab / N, b = ab - a * N;
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    const int t = s * NT + tid, l = t / N3;
-    if (t < EPB * N3) {
-      const long es = lb * EPB + l;
-      int v = -1;
-      if (es < ne) {
-        const long e = elist ? elist[es] : es;
-        v = enodes[e * N3 + (t - l * N3)];
-      }
-      IDX[t] = v;
-    }
-  }
   for (int t = tid; t < EPB * 8; t += NT) {
     const int l = t >> 3, vt = t & 7;
     const long es = lb * EPB + l;
     double *dst = &VX[(l * 8 + vt) * 4];
     if (es < ne) {
-      const long e = elist ? elist[es] : es;
-      const long vi = everts[e * 8 + vt];
+      const int e = elist ? elist[es] : (int)es;
+      int ex, ey, ez;
+      elem_coords(bm, e, ex, ey, ez);
+      const long vi = vertex_index(bm, ex, ey, ez, vt);
       dst[0] = vxyz[3 * vi];
       dst[1] = vxyz[3 * vi + 1];
       dst[2] = vxyz[3 * vi + 2];
       dst[3] = ph.has_k ? kv[vi] : 1.0;
+      if (vt == 0) {
+        EB[4 * l] = ex * bm.p;
+        EB[4 * l + 1] = ey * bm.p;
+        EB[4 * l + 2] = ez * bm.p;
+        EB[4 * l + 3] = 1;
+      }
     } else {  // unit cube for empty slots: finite geometry, nothing is scattered
       dst[0] = vt >> 2;
       dst[1] = (vt >> 1) & 1;
       dst[2] = vt & 1;
       dst[3] = 1.0;
+      if (vt == 0) EB[4 * l + 3] = 0;
     }
   }
   __syncthreads();
-  // gather of the first field, coalesced over the element node lists (tile order = node order);
-  // the Jacobian action prefetches the state into registers behind the direction's contractions
+  // gather of the first field, coalesced over the element's nodes (tile order = node order); the
+  // Jacobian action prefetches the state into registers behind the direction's contractions
   T upre[MODE == MODE_JAC ? NS : 1];
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int t = s * NT + tid;
     if (t < EPB * N3) {
-      const int c = IDX[t];
-      T v = 0;
-      if (MODE == MODE_FUN) v = c >= 0 ? ustate[c] : T(0);
-      else v = (c >= 0 && c < nowned) ? xdir[c] : T(0);
-      TA[t] = v;
-      if (MODE == MODE_JAC) upre[s] = c >= 0 ? ustate[c] : T(0);
+      const int l = t / N3;
+      int own = -1;
+      T v = 0, uv = 0;
+      if (EB[4 * l + 3]) {
+        int I = EB[4 * l], J = EB[4 * l + 1], K = EB[4 * l + 2];
+        node_lattice<N>(t - l * N3, I, J, K);
+        own = owned_index(bm, I, J, K);
+        if (MODE != MODE_LIN) uv = ustate[(I * bm.Ly + J) * bm.Lz + K];
+        if (MODE != MODE_FUN && own >= 0) v = xdir[own];
+      }
+      IDX[t] = own;
+      TA[t] = MODE == MODE_FUN ? uv : v;
+      if (MODE == MODE_JAC) upre[s] = uv;
     }
   }
   __syncthreads();
@@ -107,10 +144,11 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__
 // indices and vertices (and, for the linear and residual modes, its gathered values) into
 // registers while computing the current one.
 template <int N, typename T, int MODE, int EPB>
-__global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_persistent_kernel(
-    long nbatch, long ne, const int *__restrict__ enodes, const int *__restrict__ everts,
-    const double *__restrict__ vxyz, const double *__restrict__ kv, const T *__restrict__ ustate,
-    const T *__restrict__ xdir, T *__restrict__ y, long nowned, const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph)
+__global__ void __launch_bounds__(wg_threads(N, EPB))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1))) ellip_persistent_kernel(
+    long nbatch, long ne, BoxMap bm, const double *__restrict__ vxyz, const double *__restrict__ kv,
+    const T *__restrict__ ustate, const T *__restrict__ xdir, T *__restrict__ y, long nowned,
+    const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph)
 {
   constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
   constexpr bool PREFETCH_VALUES = MODE != MODE_JAC;
@@ -123,16 +161,24 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_persistent_kernel(
   const long hi = nbatch * (xcd + 1) / 8;
   long bt = nbatch * xcd / 8 + w;
   constexpr int NV = (EPB * 32 + NT - 1) / NT;  // vertex values (x, y, z, k of 8 vertices) per lane
-  int pidx[NS];
+  int pidx[PREFETCH_VALUES ? NS : 1];
   T pval[PREFETCH_VALUES ? NS : 1];
   double pvx[NV];
-  auto load_idx = [&](long bb) {
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      const int t = s * NT + tid, l = t / N3;
-      const long es = bb * EPB + l;
-      pidx[s] = (t < EPB * N3 && es < ne) ? enodes[es * N3 + (t - l * N3)] : -1;
+  // node indices of slot t of batch bb: owned index (-1 if none) and lattice index (-1 past the end)
+  auto slot_nodes = [&](long bb, int t, int &own, int &lat) {
+    const int l = t / N3;
+    const long es = bb * EPB + l;
+    own = lat = -1;
+    if (t < EPB * N3 && es < ne) {
+      int ex, ey, ez;
+      elem_coords(bm, (int)es, ex, ey, ez);
+      int I = ex * bm.p, J = ey * bm.p, K = ez * bm.p;
+      node_lattice<N>(t - l * N3, I, J, K);
+      own = owned_index(bm, I, J, K);
+      lat = (I * bm.Ly + J) * bm.Lz + K;
     }
+  };
+  auto load_idx = [&](long bb) {
 #pragma unroll
     for (int q = 0; q < NV; q++) {
       const int t = q * NT + tid;
@@ -140,7 +186,9 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_persistent_kernel(
         const int l = t >> 5, vt = (t >> 2) & 7, d = t & 3;
         const long es = bb * EPB + l;
         if (es < ne) {
-          const long vi = everts[es * 8 + vt];
+          int ex, ey, ez;
+          elem_coords(bm, (int)es, ex, ey, ez);
+          const long vi = vertex_index(bm, ex, ey, ez, vt);
           pvx[q] = d < 3 ? vxyz[3 * vi + d] : (ph.has_k ? kv[vi] : 1.0);
         } else {  // unit cube for empty slots: finite geometry, nothing is scattered
           pvx[q] = d == 3 ? 1.0 : (double)((vt >> (2 - d)) & 1);
@@ -148,45 +196,54 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_persistent_kernel(
       }
     }
   };
-  auto load_val = [&]() {
+  auto load_val = [&](long bb) {
     if constexpr (PREFETCH_VALUES) {
+      int tl = tid;  // slot positions are recomputed per batch, not kept live across the loop
+      asm volatile("" : "+v"(tl));
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        const int c = pidx[s];
-        pval[s] = MODE == MODE_FUN ? (c >= 0 ? ustate[c] : T(0)) : ((c >= 0 && c < nowned) ? xdir[c] : T(0));
+        int own, lat;
+        slot_nodes(bb, s * NT + tl, own, lat);
+        pidx[s] = own;
+        pval[s] = MODE == MODE_FUN ? (lat >= 0 ? ustate[lat] : T(0)) : (own >= 0 ? xdir[own] : T(0));
       }
     }
   };
   if (bt < hi) {
     load_idx(bt);
-    load_val();
+    load_val(bt);
   }
   for (; bt < hi; bt += per) {
+    if constexpr (PREFETCH_VALUES) {
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      const int t = s * NT + tid;
-      if (t < EPB * N3) {
-        IDX[t] = pidx[s];
-        if constexpr (PREFETCH_VALUES) TA[t] = pval[s];
+      for (int s = 0; s < NS; s++) {
+        const int t = s * NT + tid;
+        if (t < EPB * N3) {
+          IDX[t] = pidx[s];
+          TA[t] = pval[s];
+        }
       }
     }
 #pragma unroll
     for (int q = 0; q < NV; q++)
       if (q * NT + tid < EPB * 32) VX[q * NT + tid] = pvx[q];
-    __syncthreads();
     T upre[MODE == MODE_JAC ? NS : 1];
     if constexpr (!PREFETCH_VALUES) {
+      int tl = tid;
+      asm volatile("" : "+v"(tl));
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        const int t = s * NT + tid;
+        const int t = s * NT + tl;
+        int own, lat;
+        slot_nodes(bt, t, own, lat);
         if (t < EPB * N3) {
-          const int c = IDX[t];
-          TA[t] = (c >= 0 && c < nowned) ? xdir[c] : T(0);
-          upre[s] = c >= 0 ? ustate[c] : T(0);
+          IDX[t] = own;
+          TA[t] = own >= 0 ? xdir[own] : T(0);
+          upre[s] = lat >= 0 ? ustate[lat] : T(0);
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
     const long nxt = bt + per;
     if (nxt < hi) load_idx(nxt);
     // a copy of the thread index the compiler cannot hoist: everything derived from it is
@@ -196,7 +253,7 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_persistent_kernel(
     const int atomic = 1;
 #define DOHP_BATCH_HOOK() \
   do {                    \
-    if (nxt < hi) load_val(); \
+    if (nxt < hi) load_val(nxt); \
   } while (0)
 #define DOHP_SCATTER(c, val) \
   do {                       \
@@ -219,12 +276,15 @@ __global__ void zero_kernel(T *__restrict__ y, long n)
   if (i < n) y[i] = T(0);
 }
 
-__global__ void closure_kernel(const double *__restrict__ u, const double *__restrict__ dcache, long nowned,
-                               long nclosure, double *__restrict__ st, float *__restrict__ st32)
+// Linearization state in lattice order: owned values from u, Dirichlet values from the cache.
+__global__ void lattice_state_kernel(const double *__restrict__ u, const double *__restrict__ dcache,
+                                     const int *__restrict__ lat2clo, long nowned, long nlat, double *__restrict__ st,
+                                     float *__restrict__ st32)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nclosure) return;
-  const double v = i < nowned ? u[i] : dcache[i - nowned];
+  if (i >= nlat) return;
+  const long c = lat2clo[i];
+  const double v = c < nowned ? u[c] : dcache[c - nowned];
   st[i] = v;
   st32[i] = (float)v;
 }
@@ -238,7 +298,7 @@ struct _p_dohpEllip {
   bool has_state = false;
   int shape[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // tuned launch shape per [mode][FP32], -1 = not yet
   float shape_ms[3][2][3] = {};
-  DevBuf<double> state, kvert, tab64, wphi;  // wphi: Gauss weights then (1 + z_q)/2
+  DevBuf<double> state, kvert, tab64, wphi;  // state in lattice order; wphi: Gauss weights then (1 + z_q)/2
   DevBuf<float> state32, tab32;
   // host tables
   std::vector<double> B, Dh, w, phi;
@@ -269,6 +329,27 @@ Phys make_phys(const _p_dohpEllip *op)
   return ph;
 }
 
+BoxMap box_map(const _p_dohpFS *fs)
+{
+  BoxMap m;
+  const int L[3] = {fs->p * fs->M[0] + 1, fs->p * fs->M[1] + 1, fs->p * fs->M[2] + 1};
+  const int walls = fs->dirichlet_mask & 1, top = (fs->dirichlet_mask >> 1) & 1, bottom = (fs->dirichlet_mask >> 2) & 1;
+  m.p = fs->p;
+  m.M1 = fs->M[1];
+  m.M2 = fs->M[2];
+  m.V1 = fs->M[1] + 1;
+  m.V2 = fs->M[2] + 1;
+  m.Ly = L[1];
+  m.Lz = L[2];
+  m.I0 = walls;
+  m.J0 = walls;
+  m.K0 = bottom;
+  m.nx = L[0] - 2 * walls;
+  m.ny = L[1] - 2 * walls;
+  m.nz = L[2] - bottom - top;
+  return m;
+}
+
 template <int N, typename T, int MODE, int EPB>
 void launch_shape(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st, const Phys &ph)
 {
@@ -277,15 +358,15 @@ void launch_shape(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStr
   if (op->scatter == 1) {
     const long ne = fs->nelem;
     hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB>), dim3((ne + EPB - 1) / EPB), block, 0, st, nullptr, ne,
-                       fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr, fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y,
-                       fs->nowned, 1, gtab<T>(op), op->wphi.ptr, ph);
+                       box_map(fs), fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 1, gtab<T>(op),
+                       op->wphi.ptr, ph);
   } else {
     for (int col = 0; col < 8; col++) {
       const long ne = fs->color_off[col + 1] - fs->color_off[col];
       if (!ne) continue;
       hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB>), dim3((ne + EPB - 1) / EPB), block, 0, st,
-                         fs->d_color_elems.ptr + fs->color_off[col], ne, fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr,
-                         fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), op->wphi.ptr, ph);
+                         fs->d_color_elems.ptr + fs->color_off[col], ne, box_map(fs), fs->d_vcoords.ptr,
+                         op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), op->wphi.ptr, ph);
     }
   }
 }
@@ -307,8 +388,8 @@ dErr launch_persistent(const _p_dohpEllip *op, const T *ust, const T *x, T *y, h
   grid = std::min(grid, ((nbatch + 7) / 8) * 8);
   grid = std::max(8L, grid - grid % 8);
   hipLaunchKernelGGL((ellip_persistent_kernel<N, T, MODE, EPB>), dim3((unsigned)grid), dim3(threads), 0, st, nbatch,
-                     (long)fs->nelem, fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr, fs->d_vcoords.ptr, op->kvert.ptr, ust,
-                     x, y, fs->nowned, gtab<T>(op), op->wphi.ptr, ph);
+                     (long)fs->nelem, box_map(fs), fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned,
+                     gtab<T>(op), op->wphi.ptr, ph);
   return 0;
 }
 
@@ -495,8 +576,8 @@ extern "C" dErr dohpEllipFunction(dohpEllip op, const double *u, double *f, void
   DOHP_CHECK(op && u && f, dohpERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
   const dohpFS fs = op->fs;
-  hipLaunchKernelGGL(closure_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
-                     fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
+  hipLaunchKernelGGL(lattice_state_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
+                     fs->d_lat2clo.ptr, fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
   op->has_state = true;
   DOHP_CALL(zero(f, fs->nowned, st));
   return launch<double, MODE_FUN>(op, op->state.ptr, nullptr, f, st);
@@ -505,8 +586,8 @@ extern "C" dErr dohpEllipFunction(dohpEllip op, const double *u, double *f, void
 extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hipStream_t st)
 {
   const dohpFS fs = op->fs;
-  hipLaunchKernelGGL(closure_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
-                     fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
+  hipLaunchKernelGGL(lattice_state_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
+                     fs->d_lat2clo.ptr, fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
   op->has_state = true;
   DOHP_HIP(hipGetLastError());
   return 0;

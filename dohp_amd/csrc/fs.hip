@@ -108,7 +108,9 @@ extern "C" dErr dFSCreateBlock(int M0, int M1, int M2, int degree, int dmask, co
     fs->trav_elems.resize(nel);
     for (long k = 0; k < nel; k++) fs->trav_elems[k] = key[k].second;
   }
+  fs->lat2clo = std::move(l2c);
   dErr err = fs->d_elem_nodes.upload(fs->elem_nodes.data(), fs->elem_nodes.size());
+  if (!err) err = fs->d_lat2clo.upload(fs->lat2clo.data(), fs->lat2clo.size());
   if (!err) err = fs->d_trav_elems.upload(fs->trav_elems.data(), fs->trav_elems.size());
   if (!err) err = fs->d_elem_verts.upload(fs->elem_verts.data(), fs->elem_verts.size());
   if (!err) err = fs->d_color_elems.upload(fs->color_elems.data(), fs->color_elems.size());

@@ -34,6 +34,7 @@ struct Q1Tab {
 struct Q1Phys {
   double p, eps2, lambda;
   int has_k;
+  int M1, M2, Ly, Lz;  // block extents: elements, and node lattice (the state is in lattice order)
 };
 
 __device__ inline void jac_at(const double (&X)[8][3], double x0, double x1, double x2, double (&J)[3][3])
@@ -102,6 +103,7 @@ __global__ void q1_assemble_kernel(int nelem, const int *__restrict__ enodes, co
     kvx[t] = ph.has_k ? kv[vi] : 1.0;
   }
   const int *en = &enodes[e * n * n * n];
+  const long e2 = e % ph.M2, e1 = (e / ph.M2) % ph.M1, e0 = e / ((long)ph.M1 * ph.M2);
   // u and reference gradient at the sub-cell's 8 patch points, sum-factorized:
   // contract c for the two kk, then b for the two jj, then a for the two ii.
   double Tz[kMaxP + 1][kMaxP + 1][2][2];  // [a][b][kk][B|D]
@@ -109,7 +111,7 @@ __global__ void q1_assemble_kernel(int nelem, const int *__restrict__ enodes, co
     for (int b = 0; b < n; b++) {
       double s[2][2] = {{0, 0}, {0, 0}};
       for (int c = 0; c < n; c++) {
-        const double uv = ustate[en[(a * n + b) * n + c]];
+        const double uv = ustate[(((int)e0 * p + a) * ph.Ly + (int)e1 * p + b) * ph.Lz + (int)e2 * p + c];
         for (int kk = 0; kk < 2; kk++) {
           s[kk][0] += tb.Bs[2 * pc + kk][c] * uv;
           s[kk][1] += tb.Ds[2 * pc + kk][c] * uv;
@@ -346,7 +348,8 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
       tb.Bs[q][a] = Bs[q * fs->n + a];
       tb.Ds[q][a] = Ds[q * fs->n + a];
     }
-  Q1Phys ph{prm.p, prm.epsilon * prm.epsilon, prm.lambda, kvert != nullptr};
+  Q1Phys ph{prm.p, prm.epsilon * prm.epsilon, prm.lambda, kvert != nullptr, fs->M[1], fs->M[2],
+            fs->p * fs->M[1] + 1, fs->p * fs->M[2] + 1};
   hipLaunchKernelGGL(zero_f64_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->nnz);
   const long nt = (long)fs->nelem * fs->p * fs->p * fs->p;
   hipLaunchKernelGGL(q1_assemble_kernel, dim3((nt + 127) / 128), dim3(128), 0, st, fs->nelem, fs->d_elem_nodes.ptr,

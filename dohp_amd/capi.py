@@ -412,7 +412,8 @@ class Ellip:
               stream=None):
         """CG on the linear operator, preconditioned through the embedded-Q1 matrix: pc="mg" is a
         multigrid V-cycle with Jacobi-Chebyshev smoothing, pc="cheb" Jacobi-Chebyshev alone.
-        mlp: learned patch correction after the V-cycle (flexible CG, plain fallback after
+        mixed: 0 plain FP64, 1 FP32 inner PCG inside FP64 defect correction, 2 FP64 PCG with the
+        preconditioner in FP32.  mlp: learned node weights around the V-cycle (plain fallback after
         fallback_its iterations).  record=(snap, every): plain FP64 solve keeping iterates."""
         o = SolveOpts(int(mixed), float(rtol), int(max_it), float(inner_rtol), int(max_outer), int(cheb_degree),
                       float(cheb_ratio), int(eig_its), {"cheb": 0, "mg": 1}[pc], int(mg_degree), int(fallback_its))

@@ -9,6 +9,8 @@
 //   * Mixed path: FP64 outer defect correction x += A32^{-1}(b - A64 x) with the inner PCG in
 //     FP32 (FP32 operator and Q1 applies); residuals of the outer loop and every Krylov dot
 //     product accumulate in FP64.
+//   * FP32-preconditioner path: FP64 PCG whose preconditioner (V-cycle or Chebyshev, learned
+//     weights included) runs in FP32.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -98,6 +100,11 @@ __global__ void promote_add_kernel(long n, double s, const float *__restrict__ y
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] += s * (double)y[i];
 }
+__global__ void promote_kernel(long n, const float *__restrict__ y, double *__restrict__ x)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (double)y[i];
+}
 
 inline dim3 grid(long n) { return dim3((n + 255) / 256); }
 
@@ -178,7 +185,8 @@ __global__ void scale2_kernel(long n, const double *__restrict__ s, const void *
 }
 
 // Preconditioner of one precision: Jacobi-Chebyshev on the Q1 matrix, or a multigrid V-cycle on
-// it, optionally corrected by learned node weights, z = S M(S r).
+// it, optionally corrected by learned node weights, z = S M(S r).  An FP64 preconditioner may hand
+// the work to an FP32 one (low): r is rounded to FP32, z comes back exactly.
 template <typename T>
 struct Prec {
   Cheb<T> *cheb = nullptr;
@@ -186,6 +194,8 @@ struct Prec {
   const double *s = nullptr;  // learned node weights, or null
   T *tmp = nullptr;
   bool learned_on = false;
+  Prec<float> *low = nullptr;
+  float *lr = nullptr, *lz = nullptr;
   dErr apply(Ctx &c, const T *r, T *z);
 };
 template <typename T>
@@ -197,6 +207,15 @@ dErr mg_apply<float>(dohpMG mg, const float *r, float *z, hipStream_t st) { retu
 template <typename T>
 dErr Prec<T>::apply(Ctx &c, const T *r, T *z)
 {
+  if constexpr (sizeof(T) == 8) {
+    if (low) {
+      hipLaunchKernelGGL(demote_kernel, grid(c.n), dim3(256), 0, c.st, c.n, 1.0, r, lr);
+      DOHP_CALL(low->apply(c, lr, lz));
+      hipLaunchKernelGGL(promote_kernel, grid(c.n), dim3(256), 0, c.st, c.n, lz, z);
+      DOHP_HIP(hipGetLastError());
+      return 0;
+    }
+  }
   if (!mg) return cheb->apply(c, r, z);
   if (!(s && learned_on)) return mg_apply<T>(mg, r, z, c.st);
   const int f32 = sizeof(T) == 4;
@@ -268,8 +287,9 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double
   double rz = 0;
   bool restart = false;
   auto precondition = [&]() -> dErr {
-    if (M.learned_on && bud.fallback_its > 0 && bud.used >= bud.fallback_its) {
-      M.learned_on = false;
+    bool &learned_on = M.low ? M.low->learned_on : M.learned_on;
+    if (learned_on && bud.fallback_its > 0 && bud.used >= bud.fallback_its) {
+      learned_on = false;
       bud.fell_back = 1;
       restart = true;
     }
@@ -385,7 +405,32 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
   }
   const auto t_setup = std::chrono::steady_clock::now();
   stats->setup_seconds = std::chrono::duration<double>(t_setup - t0).count();
-  if (!opts->mixed) {
+  if (opts->mixed == 2) {  // FP64 PCG, preconditioner in FP32
+    Cheb<float> C32{q1, dinv32.ptr, lmin, lmax, degree};
+    DOHP_CALL(C32.res.alloc(c.n));
+    DOHP_CALL(C32.d.alloc(c.n));
+    DOHP_CALL(C32.w.alloc(c.n));
+    Prec<float> M32;
+    M32.cheb = &C32;
+    M32.mg = mgh.mg;
+    M32.s = wts.ptr;
+    M32.tmp = tmp32.ptr;
+    M32.learned_on = mlp != nullptr;
+    DevBuf<float> lr, lz;
+    DOHP_CALL(lr.alloc(c.n));
+    DOHP_CALL(lz.alloc(c.n));
+    Prec<double> M;
+    M.low = &M32;
+    M.lr = lr.ptr;
+    M.lz = lz.ptr;
+    int its;
+    double rn, bn;
+    DOHP_CALL(pcg<double>(c, op, M, bud, b, x, opts->rtol, opts->max_it, &its, &rn, &bn, snap, nsnap, every, nsnapped));
+    stats->inner_its = its;
+    stats->outer_its = 1;
+    stats->rnorm = rn;
+    stats->bnorm = bn;
+  } else if (!opts->mixed) {
     Cheb<double> C64{q1, dinv.ptr, lmin, lmax, degree};
     DOHP_CALL(C64.res.alloc(c.n));
     DOHP_CALL(C64.d.alloc(c.n));

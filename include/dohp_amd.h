@@ -246,7 +246,8 @@ dErr dohpMLPEnergyStep(dohpMLP mlp, dohpEllip op, dohpMat q1, dohpMG mg, const f
 
 /* ------------------------------------------------------------------ solvers */
 typedef struct {
-  int mixed;          /* 0: plain FP64 PCG; 1: FP32 inner PCG inside FP64 defect correction */
+  int mixed;          /* 0: plain FP64 PCG; 1: FP32 inner PCG inside FP64 defect correction;
+                         2: FP64 PCG with the preconditioner applied in FP32 */
   double rtol;        /* relative residual target ||b - A x|| <= rtol ||b|| (FP64 residual) */
   int max_it;         /* PCG iteration limit (per inner solve when mixed) */
   double inner_rtol;  /* inner FP32 PCG tolerance (mixed) */

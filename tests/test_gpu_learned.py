@@ -102,8 +102,9 @@ def test_training_and_learned_solves(C, small):
         net = C.MLP(init_params(C.MLP.nparam))
     fs, op, q1, b, xs = small
     res = {}
-    for name, kw in [("plain", {}), ("learned", {"mlp": net}), ("learned_mixed", {"mlp": net, "mixed": True}),
-                     ("fallback", {"mlp": net, "fallback_its": 2})]:
+    for name, kw in [("plain", {}), ("learned", {"mlp": net}), ("learned_mixed", {"mlp": net, "mixed": 1}),
+                     ("learned_f32_precond", {"mlp": net, "mixed": 2}), ("fallback", {"mlp": net, "fallback_its": 2}),
+                     ("fallback_f32_precond", {"mlp": net, "fallback_its": 2, "mixed": 2})]:
         x = torch.zeros_like(b)
         st = op.solve(q1, b, x, rtol=1e-10, pc="mg", **kw)
         r = torch.zeros_like(b)
@@ -113,3 +114,4 @@ def test_training_and_learned_solves(C, small):
         assert relres <= 1.01e-10
         res[name] = st
     assert res["fallback"]["fallback"] == 1 and res["learned"]["fallback"] == 0
+    assert res["fallback_f32_precond"]["fallback"] == 1 and res["learned_f32_precond"]["fallback"] == 0

@@ -99,7 +99,7 @@ def test_solve_plain_and_mixed_reach_1e10(C, M, deg):
     fs, op, q1, xs, b = _problem(C, M, deg)
     its = {}
     for pc in ("cheb", "mg"):
-        for mixed in (False, True):
+        for mixed in (0, 1, 2):
             x = torch.zeros_like(xs)
             st = op.solve(q1, b, x, mixed=mixed, rtol=1e-10, pc=pc)
             r = torch.zeros_like(xs)
@@ -110,4 +110,6 @@ def test_solve_plain_and_mixed_reach_1e10(C, M, deg):
             assert relres <= 1.01e-10
             assert err < 1e-6
             its[(pc, mixed)] = st["inner_its"]
-    assert its[("mg", False)] < its[("cheb", False)]
+    assert its[("mg", 0)] < its[("cheb", 0)]
+    # an FP32 preconditioner inside FP64 CG costs at most a few iterations
+    assert its[("mg", 2)] <= its[("mg", 0)] + 3

@@ -16,9 +16,11 @@ profile summary committed under profiles/ when it covers this workload.
 
 --workload solve: variable-coefficient Poisson -div(k grad u) = f with log-normal k (unit
 variance, correlation length 0.1, seeded) on 48^3 elements, order 5; CG preconditioned by the
-embedded-Q1 matrix (multigrid V-cycle on it with Jacobi-Chebyshev smoothing), plain FP64 versus FP32-inner/FP64-outer defect
-correction, both to relative residual 1e-10; prints one JSON line with iterations, times and
-the final FP64 error against the manufactured solution.
+embedded-Q1 matrix (multigrid V-cycle on it with Jacobi-Chebyshev smoothing), plain FP64 versus
+FP32-inner/FP64-outer defect correction and FP64 CG with the preconditioner in FP32, all to
+relative residual 1e-10, with the embedded-Q1 matrix in the reference's arithmetic and corrected
+(runs named corrected_q1_*); prints one JSON line with iterations, times and the final FP64 error
+against the manufactured solution.  `value` is the reference-arithmetic mixed solve.
 
 --workload learned: the same problem family on 48^3 elements, order 6.  The learned node weights
 of the Q1 multigrid preconditioner (3-layer MLP per 2x2x2 patch, bfloat16 weights, FP32
@@ -152,14 +154,19 @@ def solve_workload(args):
     t_asm = time.perf_counter() - t_asm
     print(f"# solve setup {time.perf_counter() - t0:.1f}s (Q1 assembly {t_asm * 1e3:.1f} ms): {fs.nelem} elements "
           f"p={p}, N_int={fs.nowned}, nnz={q1.nnz}", file=sys.stderr, flush=True)
+    op.set_q1_mode(1)
+    q1c = op.assemble_q1(torch.zeros_like(xs))
+    op.set_q1_mode(0)
     res = {}
-    for name, mixed in (("plain_f64", 0), ("mixed_f32_f64", 1), ("f64_cg_f32_precond", 2)):
+    for name, mixed, qq in (("plain_f64", 0, q1), ("mixed_f32_f64", 1, q1), ("f64_cg_f32_precond", 2, q1),
+                            ("corrected_q1_plain_f64", 0, q1c), ("corrected_q1_mixed_f32_f64", 1, q1c),
+                            ("corrected_q1_f64_cg_f32_precond", 2, q1c)):
         x = torch.zeros_like(xs)
-        op.solve(q1, b, x, mixed=mixed, rtol=1e-10)  # warm-up (kernels, allocations)
+        op.solve(qq, b, x, mixed=mixed, rtol=1e-10)  # warm-up (kernels, allocations)
         x.zero_()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        st = op.solve(q1, b, x, mixed=mixed, rtol=1e-10)
+        st = op.solve(qq, b, x, mixed=mixed, rtol=1e-10)
         torch.cuda.synchronize()
         t = time.perf_counter() - t
         r = torch.zeros_like(x)
@@ -176,6 +183,8 @@ def solve_workload(args):
            "speedup_mixed_vs_plain": round(res["plain_f64"]["seconds"] / res["mixed_f32_f64"]["seconds"], 3),
            "fastest_reduced_precision": best,
            "speedup_fastest_vs_plain": round(res["plain_f64"]["seconds"] / res[best]["seconds"], 3),
+           "q1_modes": "runs named corrected_q1_* precondition with the corrected embedded-Q1 matrix (sub-cell "
+                       "derivatives 1/h, inverse Jacobian at the point); the others with the reference's arithmetic",
            "runs": res}
     print(json.dumps(out))
 

@@ -68,6 +68,7 @@ _proto("dohpEllipShellMatMultAdd", _vp, _vp, _vp, _vp)
 _proto("dohpEllipSetScatter", _vp, ctypes.c_int)
 _proto("dohpEllipGetTuning", _vp, ctypes.c_int, ctypes.c_int, _vp, _vp)
 _proto("dohpEllipAssembleQ1", _vp, _vp, ctypes.POINTER(_vp), _vp)
+_proto("dohpEllipSetQ1Mode", _vp, ctypes.c_int)
 _proto("dohpMatGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long))
 _proto("dohpMatGetCSR", _vp, _ip, _ip, _dp)
 _proto("dohpMatMult", _vp, _vp, _vp, _vp)
@@ -386,6 +387,10 @@ class Ellip:
 
     def set_scatter(self, mode):
         check(_lib.dohpEllipSetScatter(self.h, int(mode)))
+
+    def set_q1_mode(self, mode):
+        """Later embedded-Q1 assemblies: 0 = the reference's arithmetic, 1 = corrected (GLL sub-mesh Q1 FEM)."""
+        check(_lib.dohpEllipSetQ1Mode(self.h, int(mode)))
 
     def function(self, u, f, stream=None):
         check(_lib.dohpEllipFunction(self.h, _ptr(u), _ptr(f), _stream(stream)))

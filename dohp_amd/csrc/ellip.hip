@@ -295,6 +295,7 @@ struct _p_dohpEllip {
   dohpFS fs = nullptr;
   dohpEllipParam prm{};
   int scatter = 1;  // 0 = colours (deterministic), 1 = atomics
+  int q1mode = 0;   // embedded-Q1 assembly: 0 = the reference's arithmetic, 1 = corrected sub-cell derivatives
   bool has_state = false;
   int shape[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // tuned launch shape per [mode][FP32], -1 = not yet
   float shape_ms[3][2][3] = {};
@@ -594,13 +595,21 @@ extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hi
 }
 
 extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpEllipParam *prm, const double **kvert,
-                                              const double **state)
+                                              const double **state, int *q1mode)
 {
   DOHP_CHECK(op, dohpERR_ARG, "null operator");
   *fs = op->fs;
   *prm = op->prm;
   *kvert = op->kvert.ptr;
   *state = op->state.ptr;
+  if (q1mode) *q1mode = op->q1mode;
+  return 0;
+}
+
+extern "C" dErr dohpEllipSetQ1Mode(dohpEllip op, int mode)
+{
+  DOHP_CHECK(op && (mode == 0 || mode == 1), dohpERR_ARG, "Q1 mode must be 0 (reference) or 1 (corrected)");
+  op->q1mode = mode;
   return 0;
 }
 

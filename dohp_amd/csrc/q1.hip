@@ -9,6 +9,8 @@
 //     patch*8+q, because the iterator passes its element-ordered array
 //     (src/fs/interface/fsrulesetit.c:684) to a routine that indexes it in patch order;
 //   * u, grad u and the weights are taken at the correct patch points.
+// dohpEllipSetQ1Mode(op, 1) assembles the corrected matrix instead (the Q1 finite-element stiffness of
+// the GLL sub-mesh): sub-cell derivatives +-1/h and the inverse Jacobian at each patch point.
 // The 8x8 sub-cell matrices are added into a CSR matrix over owned nodes (Dirichlet rows and
 // columns are dropped, src/fs/interface/fsbuild.c:31-34).  One thread per sub-cell.
 #include <algorithm>
@@ -26,6 +28,8 @@ constexpr int kMaxP = 10;  // degree limit of the sub-cell kernel (n <= 11)
 
 struct Q1Tab {
   int p, n, nq;                     // degree, nodes per direction, sparse points per direction (2p)
+  int corrected;                    // 1: sub-cell derivatives +-1/h and the inverse Jacobian at the point
+  double hsub[kMaxP];               // GLL sub-interval widths (element-reference coordinates)
   double zq[2 * kMaxP], wq[2 * kMaxP];
   double Bs[2 * kMaxP][kMaxP + 1];  // GLL basis at the sparse points
   double Ds[2 * kMaxP][kMaxP + 1];  // its derivative
@@ -167,16 +171,24 @@ __global__ void q1_assemble_kernel(int nelem, const int *__restrict__ enodes, co
           for (int t = 0; t < 8; t++) kk2 += kvx[t] * f0[t >> 2] * f1[(t >> 1) & 1] * f2[t & 1];
           kq[q] = kk2;
         }
-        // element-natural point pidx*8+q, decoded on the (2p)^3 sparse lattice
-        const int qe = pidx * 8 + q;
-        const int ei = qe / (nq * nq), ej = (qe / nq) % nq, ek = qe % nq;
-        double J2[3][3];
-        jac_at(X, tb.zq[ei], tb.zq[ej], tb.zq[ek], J2);
-        invert3(J2, cj[q]);
+        if (tb.corrected) {  // the inverse Jacobian at the point itself
+          for (int r = 0; r < 3; r++)
+            for (int d = 0; d < 3; d++) cj[q][r][d] = Ji[r][d];
+        } else {  // element-natural point pidx*8+q, decoded on the (2p)^3 sparse lattice
+          const int qe = pidx * 8 + q;
+          const int ei = qe / (nq * nq), ej = (qe / nq) % nq, ek = qe % nq;
+          double J2[3][3];
+          jac_at(X, tb.zq[ei], tb.zq[ej], tb.zq[ek], J2);
+          invert3(J2, cj[q]);
+        }
       }
   // sub-cell tables and the 8x8 block (ellip.c:507-520)
   const double i1[2][2] = {{0.78867513459481287, 0.21132486540518713}, {0.21132486540518713, 0.78867513459481287}};
   const double d1[2][2] = {{-1., 1.}, {-1., 1.}};
+  // per-direction derivative scale of the sub-cell's hat functions: 1 in the reference's arithmetic,
+  // 1/h (the sub-interval width) when corrected
+  const double sx = tb.corrected ? 1.0 / tb.hsub[pa] : 1.0, sy = tb.corrected ? 1.0 / tb.hsub[pb] : 1.0,
+               sz = tb.corrected ? 1.0 / tb.hsub[pc] : 1.0;
   double K[8][8];
   for (int r = 0; r < 8; r++)
     for (int c = 0; c < 8; c++) K[r][c] = 0.0;
@@ -195,9 +207,9 @@ __global__ void q1_assemble_kernel(int nelem, const int *__restrict__ enodes, co
     double I8[8], D8[8][3];
     for (int b = 0; b < 8; b++) {
       const int a0 = b >> 2, a1 = (b >> 1) & 1, a2 = b & 1;
-      const double dX = d1[ii][a0] * i1[jj][a1] * i1[kk][a2];
-      const double dY = i1[ii][a0] * d1[jj][a1] * i1[kk][a2];
-      const double dZ = i1[ii][a0] * i1[jj][a1] * d1[kk][a2];
+      const double dX = sx * d1[ii][a0] * i1[jj][a1] * i1[kk][a2];
+      const double dY = sy * i1[ii][a0] * d1[jj][a1] * i1[kk][a2];
+      const double dZ = sz * i1[ii][a0] * i1[jj][a1] * d1[kk][a2];
       I8[b] = i1[ii][a0] * i1[jj][a1] * i1[kk][a2];
       for (int d = 0; d < 3; d++) D8[b][d] = dX * cj[q][0][d] + dY * cj[q][1][d] + dZ * cj[q][2][d];
     }
@@ -266,7 +278,7 @@ __global__ void zero_f64_kernel(double *__restrict__ a, long n)
 
 // Accessors the operator file uses (defined there).
 extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpEllipParam *prm, const double **kvert,
-                                              const double **state);
+                                              const double **state, int *q1mode);
 
 // Pattern of the Q1 matrix on the GLL sub-cell lattice: row of every owned node, columns of the
 // owned nodes among its 27 lattice neighbours (dFSCreateMatrix preallocates 27 per row,
@@ -320,7 +332,8 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   dohpFS fs;
   dohpEllipParam prm;
   const double *kvert, *state;
-  DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kvert, &state));
+  int q1mode = 0;
+  DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kvert, &state, &q1mode));
   DOHP_CHECK(fs->p <= kMaxP, dohpERR_ARG, "Q1 assembly supports degree <= 10");
   hipStream_t st = (hipStream_t)stream;
   if (u) DOHP_CALL(dohpEllipFunctionState_Private(op, u, st));
@@ -338,6 +351,12 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   Q1Tab tb;
   tb.p = fs->p;
   tb.n = fs->n;
+  tb.corrected = q1mode;
+  {
+    double zg[kMaxN], wg[kMaxN];
+    gauss_lobatto(fs->n, zg, wg);
+    for (int i = 0; i < fs->p; i++) tb.hsub[i] = zg[i + 1] - zg[i];
+  }
   int nq;
   DOHP_CALL(dohpTensorRule(dQUADRATURE_METHOD_SPARSE, dGAUSS_LEGENDRE, 2 * fs->p, &nq, tb.zq, tb.wq));
   tb.nq = nq;

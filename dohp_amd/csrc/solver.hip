@@ -21,7 +21,7 @@
 using namespace dohp;
 
 extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpEllipParam *prm, const double **kvert,
-                                              const double **state);
+                                              const double **state, int *q1mode);
 
 namespace {
 
@@ -391,7 +391,7 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     dohpFS fs;
     dohpEllipParam prm;
     const double *kv, *stt;
-    DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kv, &stt));
+    DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kv, &stt, nullptr));
     long np;
     DOHP_CALL(dohpMLPNumPatches(q1, &np));
     DevBuf<float> X;

@@ -175,6 +175,11 @@ typedef struct _p_dohpMat *dohpMat;
  * 27 lattice neighbours, dFSCreateMatrix, src/fs/interface/fs.c:770-813); otherwise it is
  * refilled in place (SAME_NONZERO_PATTERN). */
 dErr dohpEllipAssembleQ1(dohpEllip op, const double *u_dev, dohpMat *mat, void *stream);
+/* Arithmetic of later Q1 assemblies (no reference counterpart): 0 = the reference's (default; sub-cell
+ * derivatives +-1 per element-reference unit, efstopo.c:665-706, and the element-natural inverse
+ * Jacobian index of fsrulesetit.c:684), 1 = corrected: derivatives +-1/h of the GLL sub-interval and the
+ * inverse Jacobian at each point, i.e. the Q1 finite-element stiffness of the GLL sub-mesh. */
+dErr dohpEllipSetQ1Mode(dohpEllip op, int mode);
 dErr dohpMatGetSizes(dohpMat A, long *nrows, long *nnz);
 /* Copies the CSR arrays to host buffers (rowptr: nrows+1, col/val: nnz); any may be NULL. */
 dErr dohpMatGetCSR(dohpMat A, int *rowptr_host, int *col_host, double *val_host);

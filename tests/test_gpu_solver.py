@@ -113,3 +113,58 @@ def test_solve_plain_and_mixed_reach_1e10(C, M, deg):
     assert its[("mg", 0)] < its[("cheb", 0)]
     # an FP32 preconditioner inside FP64 CG costs at most a few iterations
     assert its[("mg", 2)] <= its[("mg", 0)] + 3
+
+
+def _dense_q1(q1):
+    rp, ci, v = q1.csr()
+    return sp.csr_matrix((v, ci, rp), shape=(len(rp) - 1, len(rp) - 1))
+
+
+def test_corrected_q1_mode(C):
+    """Corrected embedded-Q1 assembly (sub-cell derivatives 1/h, inverse Jacobian at the point): at p = 1
+    it is exactly a quarter of the reference's arithmetic (one sub-cell of width 2); on affine elements of
+    any degree it integrates the energy of linear functions exactly (the reference's does not)."""
+    print("corrected q1", flush=True)
+    A = np.array([[1.0, 0.2, 0.1], [0.0, 0.9, 0.3], [0.2, 0.0, 1.1]])
+    for deg in (1, 3, 5):
+        M = (2, 3, 2)
+        V = block_vertices(M, box=(0, 1, 0, 1, 0, 1)) @ A.T
+        fs = C.FS(M, deg, V, dirichlet_mask=0)
+        op = C.Ellip(fs, p=2.0)
+        u0 = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+        K0 = _dense_q1(op.assemble_q1(u0))
+        op.set_q1_mode(1)
+        K1 = _dense_q1(op.assemble_q1(u0))
+        one = np.ones(fs.nowned)
+        kmax = abs(K1).max()
+        assert np.abs(K1 @ one).max() < 1e-12 * kmax
+        assert abs(K1 - K1.T).max() < 1e-13 * kmax
+        g = np.array([0.3, -1.1, 0.7])
+        u = fs.nodal_coordinates()[:fs.nowned] @ g
+        e_true = (g @ g) * np.linalg.det(A)
+        e1, e0 = u @ (K1 @ u), u @ (K0 @ u)
+        print(f"p={deg}: energy corrected {e1:.12f} reference arithmetic {e0:.6f} exact {e_true:.12f}")
+        assert abs(e1 - e_true) < 1e-11 * e_true
+        if deg == 1:
+            assert abs(K0 - 4 * K1).max() < 1e-12 * abs(K0).max()
+        else:
+            assert abs(e0 - e_true) > 1e-3 * e_true
+
+
+def test_corrected_q1_solves_in_fewer_iterations(C):
+    print("corrected q1 solve", flush=True)
+    fs, op, q1, xs, b = _problem(C, (6, 6, 6), 5)
+    op.set_q1_mode(1)
+    q1c = op.assemble_q1(torch.zeros_like(xs))
+    its = {}
+    for name, q in (("reference", q1), ("corrected", q1c)):
+        for mixed in (0, 2):
+            x = torch.zeros_like(xs)
+            st = op.solve(q, b, x, mixed=mixed, rtol=1e-10, pc="mg")
+            r = torch.zeros_like(xs)
+            op.mat_mult(x, r)
+            relres = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
+            print(name, mixed, st["inner_its"], relres)
+            assert relres <= 1.01e-10
+            its[(name, mixed)] = st["inner_its"]
+    assert its[("corrected", 0)] < its[("reference", 0)]

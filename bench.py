@@ -25,11 +25,12 @@ against the manufactured solution.  `value` is the reference-arithmetic mixed so
 --workload learned: the same problem family on 48^3 elements, order 6.  The learned node weights
 of the Q1 multigrid preconditioner (3-layer MLP per 2x2x2 patch, bfloat16 weights, FP32
 accumulation) are trained by the library's HIP kernels on small seeded problems (12^3, p = 6),
-selected on a held-out one, then used inside the reduced-precision solves, with the plain
+selected on a held-out 48^3 problem, then used inside the reduced-precision solves, with the plain
 preconditioner taking over after as many iterations as the plain FP64 solve needed.  Timed to
 1e-10: plain FP64, FP32-inner/FP64-outer defect correction, FP64 CG with the preconditioner in
 FP32, and the last two with the learned weights (when training produced a network).  `value` is
-the faster learned run, or without a network the faster reduced-precision run.
+the faster learned run, or without a network the faster reduced-precision run; all of it is done
+for the embedded-Q1 matrix in the reference's arithmetic and for the corrected one.
 
 --workload batch: one 32^3 hexahedral mesh (perturbed) whose elements have degrees drawn uniformly
 from 2..10 (seeded), each an independent element-local Poisson stiffness, plus 256 independent
@@ -194,184 +195,58 @@ def learned_workload(args):
     import dohp_amd.capi as C
     from dohp_amd import learned as Ld
     Me, p = args.elems or 48, args.degree or 6
-    t0 = time.perf_counter()
-    net, rep = Ld.fit(C, torch, train_seeds=[101, 102, 103], val_seeds=[201], M=(12, 12, 12), deg=p, epochs=20,
-                      lr=3e-3, lanczos=30)
-    t_train = time.perf_counter() - t0
-    print(f"# training {t_train:.1f}s: " + json.dumps({k: v for k, v in rep.items() if k != "history"}),
-          file=sys.stderr, flush=True)
-    fs, op, q1, b, xs = Ld.make_problem(C, (Me, Me, Me), p, 11, torch)
-    torch.cuda.synchronize()
-    print(f"# problem {Me}^3 p={p}: N_int={fs.nowned}, nnz={q1.nnz}", file=sys.stderr, flush=True)
-    res = {}
+    res, training = {}, {}
     plain_its = None
-    for name, kw in (("plain_f64", {}), ("mixed_f32_f64", {"mixed": 1}), ("f64_cg_f32_precond", {"mixed": 2}),
-                     ("learned_mixed_f32_f64", {"mixed": 1, "mlp": net}), ("learned_f64", {"mlp": net}),
-                     ("learned_f64_cg_f32_precond", {"mixed": 2, "mlp": net})):
-        if kw.get("mlp", 1) is None:
-            continue
-        if "mlp" in kw:
-            kw = dict(kw, fallback_its=plain_its)
-        x = torch.zeros_like(b)
-        op.solve(q1, b, x, rtol=1e-10, pc="mg", **kw)  # warm-up
-        x.zero_()
+    for q1_mode, tag in ((0, ""), (1, "corrected_q1_")):
+        t0 = time.perf_counter()
+        net, rep = Ld.fit(C, torch, train_seeds=[101, 102, 103], val_seeds=[201], M=(12, 12, 12), deg=p, epochs=20,
+                          lr=3e-3, lanczos=30, q1_mode=q1_mode, val_M=(Me, Me, Me))
+        t_train = time.perf_counter() - t0
+        training[tag or "reference_q1"] = {"seconds": round(t_train, 2), **{k: v for k, v in rep.items() if k != "history"}}
+        print(f"# training ({tag or 'reference q1'}) {t_train:.1f}s: " +
+              json.dumps({k: v for k, v in rep.items() if k != "history"}), file=sys.stderr, flush=True)
+        fs, op, q1, b, xs = Ld.make_problem(C, (Me, Me, Me), p, 11, torch, q1_mode=q1_mode)
         torch.cuda.synchronize()
-        t = time.perf_counter()
-        st = op.solve(q1, b, x, rtol=1e-10, pc="mg", **kw)
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t
-        r = torch.zeros_like(x)
-        op.mat_mult(x, r)
-        res[name] = {"seconds": round(t, 4), "outer_its": st["outer_its"], "inner_its": st["inner_its"],
-                     "fallback": st["fallback"], "relres": float(torch.linalg.norm(b - r) / torch.linalg.norm(b)),
-                     "error_vs_exact": float(torch.linalg.norm(x - xs) / torch.linalg.norm(xs))}
-        if name == "plain_f64":
-            plain_its = st["inner_its"]
-        print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
-    cands = [k for k in ("learned_mixed_f32_f64", "learned_f64_cg_f32_precond") if k in res] or \
-        ["mixed_f32_f64", "f64_cg_f32_precond"]
+        n_int = fs.nowned
+        print(f"# problem {Me}^3 p={p}: N_int={fs.nowned}, nnz={q1.nnz}", file=sys.stderr, flush=True)
+        for name, kw in (("plain_f64", {}), ("mixed_f32_f64", {"mixed": 1}), ("f64_cg_f32_precond", {"mixed": 2}),
+                         ("learned_mixed_f32_f64", {"mixed": 1, "mlp": net}), ("learned_f64", {"mlp": net}),
+                         ("learned_f64_cg_f32_precond", {"mixed": 2, "mlp": net})):
+            if kw.get("mlp", 1) is None:
+                continue
+            if "mlp" in kw:
+                kw = dict(kw, fallback_its=plain_its)
+            x = torch.zeros_like(b)
+            op.solve(q1, b, x, rtol=1e-10, pc="mg", **kw)  # warm-up
+            x.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            st = op.solve(q1, b, x, rtol=1e-10, pc="mg", **kw)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t
+            r = torch.zeros_like(x)
+            op.mat_mult(x, r)
+            key = tag + name
+            res[key] = {"seconds": round(t, 4), "outer_its": st["outer_its"], "inner_its": st["inner_its"],
+                        "fallback": st["fallback"], "relres": float(torch.linalg.norm(b - r) / torch.linalg.norm(b)),
+                        "error_vs_exact": float(torch.linalg.norm(x - xs) / torch.linalg.norm(xs))}
+            if name == "plain_f64":
+                plain_its = st["inner_its"]
+            print(f"# {key}: {res[key]}", file=sys.stderr, flush=True)
+        del fs, op, q1, b, xs
+    learned = [k for k in res if "learned" in k]
+    cands = learned or [k for k in res if k != "plain_f64" and "plain" not in k]
     best_name = min(cands, key=lambda k: res[k]["seconds"])
     best = res[best_name]
     out = {"metric": "solve time to relative residual 1e-10 (learned preconditioner)", "value": best["seconds"],
-           "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 inner / f64 outer", "data": "synthetic",
+           "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 / f64", "data": "synthetic",
            "config": {"workload": f"log-normal k Poisson, {Me}^3 hex, p={p}, CG + multigrid on the embedded Q1 with "
-                                  "learned node weights", "n_int": fs.nowned},
-           "reported_run": best_name,
+                                  "learned node weights", "n_int": n_int},
+           "reported_run": best_name, "learned_network_used": bool(learned),
            "speedup_vs_plain_f64": round(res["plain_f64"]["seconds"] / best["seconds"], 3),
-           "training": {"seconds": round(t_train, 2), **{k: v for k, v in rep.items() if k != "history"}},
-           "runs": res}
-    print(json.dumps(out))
-
-
-def batch_problem(Me=32, nprob=256):
-    """(mixed-mesh degrees, element vertices, [(M, degree, V)] of the independent problems)."""
-    from dohp_amd.mesh import block_vertices, element_vertices, uniform_integers
-    M = (Me, Me, Me)
-    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=3)
-    deg = uniform_integers(4, Me ** 3, 2, 10)
-    ns, ps = uniform_integers(6, nprob, 4, 16), uniform_integers(7, nprob, 2, 8)
-    probs = []
-    for i in range(nprob):
-        Mi = (int(ns[i]),) * 3
-        probs.append((Mi, int(ps[i]), block_vertices(Mi, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1000 + i)))
-    return deg, element_vertices(M, V), probs
-
-
-def batch_workload(args):
-    import torch
-    import dohp_amd.capi as C
-    from dohp_amd.mesh import splitmix64_uniform
-    t0 = time.perf_counter()
-    Me = args.elems or 32
-    deg, vx, probs = batch_problem(Me)
-    B = C.Batch()
-    seg_mixed = B.add_elements(deg, vx)
-    segs, nbytes = [], {4: 0, 8: 0}
-    for (Mi, pi, Vi) in probs:
-        fs = C.FS(Mi, pi, Vi)
-        segs.append(B.add_space(fs))
-        for s in (4, 8):
-            nbytes[s] += alg_bytes(s, fs.nclosure, fs.nowned, pi + 1, fs.nelem, fs.nvert, False)
-        del fs
-    nexp = int(((deg + 1) ** 3).sum())
-    for s in (4, 8):  # element-local: read x, write y, 8 vertex coordinates per element
-        nbytes[s] += 2 * s * nexp + 24 * 8 * deg.size
-    B.setup()
-    n = B.size
-    x = torch.as_tensor(2 * splitmix64_uniform(9, n) - 1, dtype=torch.float64, device="cuda")
-    x32 = x.float()
-    y, y32 = torch.zeros_like(x), torch.zeros_like(x32)
-    B.mult(x, y)          # the first application of each precision tunes the tile per degree
-    B.mult_f32(x32, y32)
-    torch.cuda.synchronize()
-    t_setup = time.perf_counter() - t0
-    nel = deg.size + sum(m[0] ** 3 for (m, _, _) in probs)
-    print(f"# setup {t_setup:.1f}s: {nel} elements, {n} packed unknowns ({nexp} element-local)",
-          file=sys.stderr, flush=True)
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        B.mult_f32(x32, y32, stream)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        B.mult_f32(x32, y32, stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    t_step = e0.elapsed_time(e1) / 1e3 / args.steps
-    t64 = time_fn(torch, lambda: B.mult(x, y, stream), max(3, args.steps // 2))
-    B.mult(x, y, stream)
-    B.mult_f32(x32, y32, stream)
-    torch.cuda.synchronize()
-    err32 = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
-    print(f"# batch fp32 {t_step * 1e6:.1f} us, fp64 {t64 * 1e6:.1f} us, fp32 vs fp64 {err32:.2e}",
-          file=sys.stderr, flush=True)
-    names = ["workgroup", "wavefront", "mfma"]
-    tuning = {}
-    for d in range(1, 11):
-        for fp32 in (0, 1):
-            tile, ms = B.tuning(d, fp32)
-            if tile >= 0:
-                tuning[f"p{d}_{'f32' if fp32 else 'f64'}"] = {"tile": names[tile], "ms": ms}
-    # FP32 with the MFMA contraction tile offered to the tuner from degree 4 up (BF16 operands, FP32
-    # accumulation; 3 = operands split into two BF16 parts), and with it fixed for every degree >= 4
-    mfma, ymf = {}, {}
-    for passes in (3, 1):
-        for forced in (False, True):
-            B.set_mfma(passes)
-            for d in range(4, 11):
-                B.set_tile(d, True, 2 if forced else -1)
-            ym = torch.zeros_like(y32)
-            B.mult_f32(x32, ym, stream)
-            tm = time_fn(torch, lambda: B.mult_f32(x32, ym, stream), max(3, args.steps // 2))
-            B.mult_f32(x32, ym, stream)
-            torch.cuda.synchronize()
-            key = f"bf16x{passes}_{'all' if forced else 'tuned'}"
-            ymf[key] = ym.double().cpu().numpy()
-            mfma[key] = {"us": round(tm * 1e6, 1), "gdofs": round(n / tm / 1e9, 3),
-                         "rel_err_vs_f64": float(torch.linalg.norm(ym.double() - y) / torch.linalg.norm(y)),
-                         "tiles": {f"p{d}": names[B.tuning(d, True)[0]] for d in range(4, 11)
-                                   if B.tuning(d, True)[0] >= 0}}
-            print(f"# batch fp32 {key}: {mfma[key]}", file=sys.stderr, flush=True)
-    B.set_mfma(0)
-    for d in range(4, 11):
-        B.set_tile(d, True, -1)
-    cpu, oracle_err = None, None
-    if not args.no_cpu:
-        from oracle import oracle as O
-        xh, yh, y32h = x.cpu().numpy(), y.cpu().numpy(), y32.double().cpu().numpy()
-        ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        ref = np.zeros(n)
-        tc = time.perf_counter()
-        o, ln = B.segment(seg_mixed)
-        ref[o:o + ln] = O.expanded_poisson_apply(deg, vx, xh[o:o + ln], nthreads=ncores)
-        for s, (Mi, pi, Vi) in zip(segs, probs):
-            o, ln = B.segment(s)
-            S = O.BlockSpace(Mi, pi, Vi)
-            E = O.Ellip(S, p=2.0, nthreads=ncores)
-            ref[o:o + ln] = E.jacobian_apply(np.zeros(S.nclosure),
-                                             np.concatenate([xh[o:o + ln], np.zeros(S.nclosure - S.nowned)]))[:ln]
-        tc = time.perf_counter() - tc
-        nr = np.linalg.norm(ref)
-        oracle_err = {"f64": float(np.linalg.norm(yh - ref) / nr), "f32": float(np.linalg.norm(y32h - ref) / nr)}
-        for key, ym in ymf.items():
-            oracle_err[f"f32_{key}"] = float(np.linalg.norm(ym - ref) / nr)
-        cpu = {"value": round(n / tc / 1e9, 4), "unit": "GDOF/s", "cores": ncores, "kind": "port",
-               "sample": "the whole batch, one oracle application (element loop over OpenMP threads)"}
-    achieved = nbytes[4] / t_step / 1e9
-    out = {"metric": "GDOF/s batched mixed-order apply (FP32)", "value": round(n / t_step / 1e9, 3), "unit": "GDOF/s",
-           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"{Me}^3 mesh of degree U{{2..10}} (element-local) + {len(probs)} Poisson problems "
-                                  "n^3, n U{4..16}, degree U{2..8}, one persistent launch",
-                      "elements": int(nel), "unknowns": int(n), "element_local_unknowns": nexp},
-           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel_us": round(t_step * 1e6, 1),
-                        "algorithmic_bytes": int(nbytes[4])},
-           "cpu_baseline": cpu,
-           "f64": {"us": round(t64 * 1e6, 1), "gdofs": round(n / t64 / 1e9, 3),
-                   "hbm_frac": round(nbytes[8] / t64 / 1e9 / HBM_PEAK_GBS, 4)},
-           "f32_rel_err_vs_f64": err32, "rel_err_vs_oracle": oracle_err, "tuning": tuning, "f32_mfma": mfma}
+           "q1_modes": "runs named corrected_q1_* use the corrected embedded-Q1 matrix (sub-cell derivatives 1/h, "
+                       "inverse Jacobian at the point); the others the reference's arithmetic",
+           "training": training, "runs": res}
     print(json.dumps(out))
 
 

@@ -20,8 +20,9 @@ from .fields import lognormal_field
 from .mesh import block_vertices, splitmix64_uniform
 
 
-def make_problem(C, M, deg, field_seed, torch, rhs_seed=5, corr_len=0.1):
-    """Log-normal Poisson problem on an M box of order deg: (fs, op, q1, b, x_exact)."""
+def make_problem(C, M, deg, field_seed, torch, rhs_seed=5, corr_len=0.1, q1_mode=0):
+    """Log-normal Poisson problem on an M box of order deg: (fs, op, q1, b, x_exact).  q1_mode 1
+    assembles the corrected embedded-Q1 matrix instead of the reference's arithmetic."""
     V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=field_seed + 1000)
     fs = C.FS(M, deg, V)
     op = C.Ellip(fs, p=2.0, kvert=lognormal_field(V, seed=field_seed, corr_len=corr_len))
@@ -29,6 +30,7 @@ def make_problem(C, M, deg, field_seed, torch, rhs_seed=5, corr_len=0.1):
                          device="cuda")
     b = torch.zeros_like(xs)
     op.mat_mult(xs, b)
+    op.set_q1_mode(q1_mode)
     q1 = op.assemble_q1(torch.zeros_like(xs))
     return fs, op, q1, b, xs
 
@@ -42,12 +44,12 @@ def problem_features(C, torch, fs, q1):
     return X
 
 
-def collect_pairs(C, torch, seeds, M=(12, 12, 12), deg=6, every=4, nsnap=8, nrandom=4):
+def collect_pairs(C, torch, seeds, M=(12, 12, 12), deg=6, every=4, nsnap=8, nrandom=4, q1_mode=0):
     """Per problem: op, q1, multigrid, patch features and residual/error pairs [(r, e)]: errors of
     plain-CG iterates (the modes CG converges on last) and seeded random errors (every mode)."""
     probs = []
     for sd in seeds:
-        fs, op, q1, b, xs = make_problem(C, M, deg, sd, torch)
+        fs, op, q1, b, xs = make_problem(C, M, deg, sd, torch, q1_mode=q1_mode)
         snap = torch.zeros((nsnap, fs.nowned), dtype=torch.float64, device="cuda")
         x = torch.zeros_like(b)
         st = op.solve(q1, b, x, pc="mg", rtol=1e-11, record=(snap, every))
@@ -163,12 +165,15 @@ def init_params(nparam, seed=0):
     return p.astype(np.float32)
 
 
-def fit(C, torch, train_seeds, val_seeds, M=(12, 12, 12), deg=6, epochs=30, lr=2e-3, seed=0, lanczos=30, nvec=3):
+def fit(C, torch, train_seeds, val_seeds, M=(12, 12, 12), deg=6, epochs=30, lr=2e-3, seed=0, lanczos=30, nvec=3,
+        q1_mode=0, val_M=None):
     """Minimize the spectral spread of S M S J on the training problems (one Adam step per problem
     and epoch, fresh Ritz vectors each time); keep the parameters with the fewest CG iterations on
-    the validation problems.  Returns (net or None if no epoch beat the plain preconditioner, report)."""
-    probs = collect_pairs(C, torch, train_seeds, M, deg, nsnap=1, nrandom=0)
-    val = collect_pairs(C, torch, val_seeds, M, deg, nsnap=1, nrandom=0)
+    the validation problems (of size val_M, default M: validating at the size the network will be
+    used at guards against weights that help small problems only).  Returns (net or None if no
+    epoch beat the plain preconditioner, report)."""
+    probs = collect_pairs(C, torch, train_seeds, M, deg, nsnap=1, nrandom=0, q1_mode=q1_mode)
+    val = collect_pairs(C, torch, val_seeds, val_M or M, deg, nsnap=1, nrandom=0, q1_mode=q1_mode)
     net = C.MLP(init_params(C.MLP.nparam, seed))
     plain_its = cg_iterations(C, torch, None, val)
     best = (sum(plain_its), None, -1)

@@ -41,7 +41,7 @@ constexpr int kPacks = 3;
 constexpr int tab_off(int N)
 {
   int s = 0;
-  for (int m = kNMin; m < N; m++) s += 4 * eo_size(m);
+  for (int m = kNMin; m < N; m++) s += kEoOps * eo_size(m);
   return s;
 }
 constexpr int wphi_off(int N) { return N * (N - 1) - 2; }  // sum of 2m over m = 2..N-1

@@ -153,7 +153,10 @@ __device__ inline float recip(float d) { return __builtin_amdgcn_rcpf(d); }
 // [P: h*h][M: h*h][Pm: h][R: h+1] (the last two only for odd n).  The coefficients are uniform
 // over the workgroup and are read with scalar loads as instruction operands.
 constexpr int eo_size(int N) { return 2 * (N / 2) * (N / 2) + (N & 1) * (2 * (N / 2) + 1); }
-enum { OP_B = 0, OP_D = 1, OP_BT = 2, OP_DT = 3 };
+// B, the collocated derivative Dh, their transposes, and the product Dh B (derivative of the
+// interpolant, one pass) with its transpose.
+enum { OP_B = 0, OP_D = 1, OP_BT = 2, OP_DT = 3, OP_DB = 4, OP_DBT = 5 };
+constexpr int kEoOps = 6;
 
 // Table loads issued after this point stay after it, so each contraction holds only its own
 // coefficients in scalar registers.
@@ -236,7 +239,9 @@ __device__ __forceinline__ void tile_sync()
 
 // Interpolation and reference gradient of the nodal tile A at the Gauss points.  On return the
 // thread holds, along its z line (q1, q2) = (a, b): values q, gradients gx, gy, gz.  A is
-// overwritten; GX and GY are scratch.  Every thread of the workgroup must call it.
+// overwritten; GX and GY are scratch.  Every thread of the workgroup must call it.  Three line
+// phases: z (B); y (B, and Dh B into GY); x (B and Dh B into GX, B of GY); the z derivative stays
+// in registers.
 template <int N, typename T, bool WS = false>
 __device__ __forceinline__ void forward(const T *tab, T *A, T *GX, T *GY, int a, int b, T (&q)[N], T (&gx)[N],
                                         T (&gy)[N], T (&gz)[N])
@@ -249,16 +254,17 @@ __device__ __forceinline__ void forward(const T *tab, T *A, T *GX, T *GY, int a,
   ld_line<N, DIR_Y>(A, a, b, in);
   eo_apply<N, 1>(tab, OP_B, in, out);
   st_line<N, DIR_Y>(A, a, b, out);
+  eo_apply<N, -1>(tab, OP_DB, in, out);
+  st_line<N, DIR_Y>(GY, a, b, out);
   tile_sync<WS>();
   ld_line<N, DIR_X>(A, a, b, in);
   eo_apply<N, 1>(tab, OP_B, in, out);
   st_line<N, DIR_X>(A, a, b, out);
-  eo_apply<N, -1>(tab, OP_D, out, in);
-  st_line<N, DIR_X>(GX, a, b, in);
-  tile_sync<WS>();
-  ld_line<N, DIR_Y>(A, a, b, in);
-  eo_apply<N, -1>(tab, OP_D, in, out);
-  st_line<N, DIR_Y>(GY, a, b, out);
+  eo_apply<N, -1>(tab, OP_DB, in, out);
+  st_line<N, DIR_X>(GX, a, b, out);
+  ld_line<N, DIR_X>(GY, a, b, in);
+  eo_apply<N, 1>(tab, OP_B, in, out);
+  st_line<N, DIR_X>(GY, a, b, out);
   tile_sync<WS>();
   ld_line<N, DIR_Z>(A, a, b, q);
   eo_apply<N, -1>(tab, OP_D, q, gz);
@@ -268,7 +274,8 @@ __device__ __forceinline__ void forward(const T *tab, T *A, T *GX, T *GY, int a,
 
 // Transposed applies: from the pointwise values v and the mapped fluxes d0, d1, d2 along the
 // thread's z line, the element's nodal contributions B^T (v + Dh_x^T d0 + Dh_y^T d1 + Dh_z^T d2)
-// land in tile A.  Ends with a barrier.
+// land in tile A.  Three line phases after the z one: x (B^T of the sum, plus (Dh B)^T of d0; B^T
+// of d1), y (B^T, plus (Dh B)^T of d1), z (B^T).  Ends with a barrier.
 template <int N, typename T, bool WS = false>
 __device__ __forceinline__ void backward(const T *tab, T *A, T *GX, T *GY, int a, int b, const T (&v)[N],
                                          const T (&d0)[N], const T (&d1)[N], const T (&d2)[N])
@@ -281,24 +288,24 @@ __device__ __forceinline__ void backward(const T *tab, T *A, T *GX, T *GY, int a
   st_line<N, DIR_Z>(GY, a, b, d1);
   st_line<N, DIR_Z>(GX, a, b, d0);
   tile_sync<WS>();
+  ld_line<N, DIR_X>(A, a, b, in);
+  eo_apply<N, 1>(tab, OP_BT, in, w);
+  ld_line<N, DIR_X>(GX, a, b, in);
+  eo_apply<N, -1>(tab, OP_DBT, in, out);
+#pragma unroll
+  for (int s = 0; s < N; s++) w[s] += out[s];
+  st_line<N, DIR_X>(A, a, b, w);
+  ld_line<N, DIR_X>(GY, a, b, in);
+  eo_apply<N, 1>(tab, OP_BT, in, out);
+  st_line<N, DIR_X>(GY, a, b, out);
+  tile_sync<WS>();
+  ld_line<N, DIR_Y>(A, a, b, in);
+  eo_apply<N, 1>(tab, OP_BT, in, w);
   ld_line<N, DIR_Y>(GY, a, b, in);
-  eo_apply<N, -1>(tab, OP_DT, in, out);
-  ld_line<N, DIR_Y>(A, a, b, w);
+  eo_apply<N, -1>(tab, OP_DBT, in, out);
 #pragma unroll
   for (int s = 0; s < N; s++) w[s] += out[s];
   st_line<N, DIR_Y>(A, a, b, w);
-  tile_sync<WS>();
-  ld_line<N, DIR_X>(GX, a, b, in);
-  eo_apply<N, -1>(tab, OP_DT, in, out);
-  ld_line<N, DIR_X>(A, a, b, w);
-#pragma unroll
-  for (int s = 0; s < N; s++) w[s] += out[s];
-  eo_apply<N, 1>(tab, OP_BT, w, out);
-  st_line<N, DIR_X>(A, a, b, out);
-  tile_sync<WS>();
-  ld_line<N, DIR_Y>(A, a, b, in);
-  eo_apply<N, 1>(tab, OP_BT, in, out);
-  st_line<N, DIR_Y>(A, a, b, out);
   tile_sync<WS>();
   ld_line<N, DIR_Z>(A, a, b, in);
   eo_apply<N, 1>(tab, OP_BT, in, out);
@@ -406,16 +413,20 @@ __global__ void min_det_kernel(int N, const int *__restrict__ everts, const doub
   mindet[e] = m;
 }
 
-// B, Dh, B^T, Dh^T of an n-point rule in the even-odd form of eo_apply, 4 * eo_size(n) entries;
-// B and Dh are [q][m] row-major.
+// B, Dh, B^T, Dh^T, Dh B, (Dh B)^T of an n-point rule in the even-odd form of eo_apply,
+// kEoOps * eo_size(n) entries; B and Dh are [q][m] row-major.
 inline std::vector<double> eo_tables(int N, const std::vector<double> &B, const std::vector<double> &Dh)
 {
   const int H = N / 2, S = eo_size(N);
-  std::vector<double> t(4 * S, 0.0);
-  for (int opi = 0; opi < 4; opi++) {
+  std::vector<double> DB(N * N, 0.0);
+  for (int q = 0; q < N; q++)
+    for (int m = 0; m < N; m++)
+      for (int j = 0; j < N; j++) DB[q * N + m] += Dh[q * N + j] * B[j * N + m];
+  std::vector<double> t(kEoOps * S, 0.0);
+  for (int opi = 0; opi < kEoOps; opi++) {
     auto A = [&](int q, int m) {
-      const std::vector<double> &M = (opi == OP_B || opi == OP_BT) ? B : Dh;
-      return (opi == OP_B || opi == OP_D) ? M[q * N + m] : M[m * N + q];
+      const std::vector<double> &M = (opi == OP_B || opi == OP_BT) ? B : (opi == OP_D || opi == OP_DT) ? Dh : DB;
+      return (opi == OP_B || opi == OP_D || opi == OP_DB) ? M[q * N + m] : M[m * N + q];
     };
     const bool sym = opi == OP_B || opi == OP_BT;
     double *T0 = &t[opi * S];

@@ -67,6 +67,7 @@ _proto("dohpEllipShellMatMultF32", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMultAdd", _vp, _vp, _vp, _vp)
 _proto("dohpEllipSetScatter", _vp, ctypes.c_int)
 _proto("dohpEllipGetTuning", _vp, ctypes.c_int, ctypes.c_int, _vp, _vp)
+_proto("dohpEllipSetShape", _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int)
 _proto("dohpEllipAssembleQ1", _vp, _vp, ctypes.POINTER(_vp), _vp)
 _proto("dohpEllipSetQ1Mode", _vp, ctypes.c_int)
 _proto("dohpMatGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long))
@@ -379,11 +380,16 @@ class Ellip:
             check(_lib.dohpEllipSetCoefficient(self.h, kp))
 
     def tuning(self, mode, fp32=False):
-        """(chosen shape, ms of [four-wavefront, one-wavefront, persistent one-wavefront])."""
-        sh, ms = ctypes.c_int(), (ctypes.c_float * 3)()
+        """(chosen shape, ms of [four-wavefront, one-wavefront, persistent one-wavefront,
+        persistent four-wavefront])."""
+        sh, ms = ctypes.c_int(), (ctypes.c_float * 4)()
         check(_lib.dohpEllipGetTuning(self.h, {"linear": 0, "residual": 1, "jacobian": 2}[mode], int(fp32),
                                       ctypes.byref(sh), ms))
         return sh.value, [round(v, 4) for v in ms]
+
+    def set_shape(self, mode, fp32, shape):
+        """Fix the launch shape of a mode ("linear", "residual", "jacobian") and precision (-1 = tune)."""
+        check(_lib.dohpEllipSetShape(self.h, {"linear": 0, "residual": 1, "jacobian": 2}[mode], int(fp32), int(shape)))
 
     def set_scatter(self, mode):
         check(_lib.dohpEllipSetScatter(self.h, int(mode)))

@@ -151,7 +151,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
     const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph)
 {
   constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
-  constexpr bool PREFETCH_VALUES = MODE != MODE_JAC;
+  // the Jacobian action prefetches its two fields only in FP32 (in FP64 the registers cost occupancy)
+  constexpr bool PREFETCH_VALUES = MODE != MODE_JAC || sizeof(T) == 4;
+  constexpr bool PREFETCH_STATE = MODE == MODE_JAC && PREFETCH_VALUES;
   __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
   __shared__ int IDX[EPB * N3];
   __shared__ double VX[EPB * 8 * 4];
@@ -162,7 +164,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
   long bt = nbatch * xcd / 8 + w;
   constexpr int NV = (EPB * 32 + NT - 1) / NT;  // vertex values (x, y, z, k of 8 vertices) per lane
   int pidx[PREFETCH_VALUES ? NS : 1];
-  T pval[PREFETCH_VALUES ? NS : 1];
+  T pval[PREFETCH_VALUES ? NS : 1], pst[PREFETCH_STATE ? NS : 1];
   double pvx[NV];
   // node indices of slot t of batch bb: owned index (-1 if none) and lattice index (-1 past the end)
   auto slot_nodes = [&](long bb, int t, int &own, int &lat) {
@@ -206,6 +208,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
         slot_nodes(bb, s * NT + tl, own, lat);
         pidx[s] = own;
         pval[s] = MODE == MODE_FUN ? (lat >= 0 ? ustate[lat] : T(0)) : (own >= 0 ? xdir[own] : T(0));
+        if constexpr (PREFETCH_STATE) pst[s] = lat >= 0 ? ustate[lat] : T(0);
       }
     }
   };
@@ -228,6 +231,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
     for (int q = 0; q < NV; q++)
       if (q * NT + tid < EPB * 32) VX[q * NT + tid] = pvx[q];
     T upre[MODE == MODE_JAC ? NS : 1];
+    if constexpr (PREFETCH_STATE) {
+#pragma unroll
+      for (int s = 0; s < NS; s++) upre[s] = pst[s];
+    }
     if constexpr (!PREFETCH_VALUES) {
       int tl = tid;
       asm volatile("" : "+v"(tl));
@@ -298,7 +305,7 @@ struct _p_dohpEllip {
   int q1mode = 0;   // embedded-Q1 assembly: 0 = the reference's arithmetic, 1 = corrected sub-cell derivatives
   bool has_state = false;
   int shape[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // tuned launch shape per [mode][FP32], -1 = not yet
-  float shape_ms[3][2][3] = {};
+  float shape_ms[3][2][4] = {};
   DevBuf<double> state, kvert, tab64, wphi;  // state in lattice order; wphi: Gauss weights then (1 + z_q)/2
   DevBuf<float> state32, tab32;
   // host tables
@@ -399,12 +406,13 @@ dErr launch_variant(const _p_dohpEllip *op, int v, const T *ust, const T *x, T *
 {
   if (v == 0) launch_shape<N, T, MODE, epb_wide(N)>(op, ust, x, y, st, ph);
   else if (v == 1) launch_shape<N, T, MODE, epb_narrow(N)>(op, ust, x, y, st, ph);
-  else DOHP_CALL((launch_persistent<N, T, MODE, epb_narrow(N)>(op, ust, x, y, st, ph)));
+  else if (v == 2) DOHP_CALL((launch_persistent<N, T, MODE, epb_narrow(N)>(op, ust, x, y, st, ph)));
+  else DOHP_CALL((launch_persistent<N, T, MODE, epb_wide(N)>(op, ust, x, y, st, ph)));
   return 0;
 }
 
 // Autotuner: the first application of each (mode, precision) times the workgroup shapes (four
-// wavefronts, one wavefront, one wavefront persistent) on a scratch output and keeps the fastest
+// wavefronts, one wavefront, and the persistent forms of both) on a scratch output and keeps the fastest
 // (the choice is part of the operator; results do not depend on it beyond the summation order
 // of the atomic scatter).
 template <int N, typename T, int MODE>
@@ -412,7 +420,7 @@ dErr launch_n(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
 {
   const Phys ph = make_phys(op);
   int &shape = op->shape[MODE][sizeof(T) == 4];
-  if (op->scatter != 1 && shape == 2) shape = -1;  // the persistent form needs the atomic scatter
+  if (op->scatter != 1 && shape >= 2) shape = -1;  // the persistent forms need the atomic scatter
   if (shape < 0) {
     const long n = op->fs->nowned;
     DevBuf<T> scratch;
@@ -421,7 +429,7 @@ dErr launch_n(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
     DOHP_HIP(hipEventCreate(&e0));
     DOHP_HIP(hipEventCreate(&e1));
     float best = 0;
-    const int nvar = op->scatter == 1 ? 3 : 2;
+    const int nvar = op->scatter == 1 ? 4 : 2;
     for (int v = 0; v < nvar; v++) {
       float ms = 0;
       for (int rep = 0; rep < 3; rep++) {  // the first run of each shape is a warm-up
@@ -558,7 +566,16 @@ extern "C" dErr dohpEllipGetTuning(dohpEllip op, int mode, int fp32, int *shape,
   DOHP_CHECK(op && mode >= 0 && mode < 3 && (fp32 == 0 || fp32 == 1), dohpERR_ARG, "bad argument");
   if (shape) *shape = op->shape[mode][fp32];
   if (ms)
-    for (int v = 0; v < 3; v++) ms[v] = op->shape_ms[mode][fp32][v];
+    for (int v = 0; v < 4; v++) ms[v] = op->shape_ms[mode][fp32][v];
+  return 0;
+}
+
+extern "C" dErr dohpEllipSetShape(dohpEllip op, int mode, int fp32, int shape)
+{
+  DOHP_CHECK(op && mode >= 0 && mode < 3 && (fp32 == 0 || fp32 == 1) && shape >= -1 && shape < 4, dohpERR_ARG,
+             "bad argument");
+  DOHP_CHECK(shape < 2 || op->scatter == 1, dohpERR_ARG, "the persistent shapes need the atomic scatter");
+  op->shape[mode][fp32] = shape;
   return 0;
 }
 

@@ -116,8 +116,11 @@ dErr dohpEllipSetScatter(dohpEllip op, int mode);
 /* Launch shape the autotuner chose on the first application of a mode (0 = linear, 1 = residual,
  * 2 = Jacobian action) and precision: -1 = not tuned yet, 0 = several elements per four-wavefront
  * workgroup, 1 = one-wavefront workgroups, 2 = persistent one-wavefront workgroups with the next
- * batch's loads in flight; ms[3] receives the measured milliseconds of each. */
+ * batch's loads in flight, 3 = persistent four-wavefront workgroups; ms[4] receives the measured
+ * milliseconds of each. */
 dErr dohpEllipGetTuning(dohpEllip op, int mode, int fp32, int *shape, float *ms);
+/* Fixes the launch shape of a mode and precision (-1 = tune again on the next application). */
+dErr dohpEllipSetShape(dohpEllip op, int mode, int fp32, int shape);
 
 /* ------------------------------------------- batched mixed-order Poisson operator */
 typedef struct _p_dohpBatch *dohpBatch;

@@ -167,6 +167,48 @@ def test_nonlinear_function_and_jacobian_action(C, M, p, lam):
     assert e32 < 1e-4
 
 
+@pytest.mark.parametrize("p", [2, 4, 6, 9])
+def test_every_launch_shape_matches_oracle(C, p):
+    """Each launch shape the tuner chooses from (four-wavefront, one-wavefront, and their persistent
+    forms), pinned in turn, for the residual, the Jacobian action and the linear apply in FP64 (1e-12)
+    and FP32 (stated error)."""
+    print(f"shapes p={p}", flush=True)
+    M = (3, 2, 3)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=17)
+    fs, S, E, op = setup(C, M, p, V, p=1.5, eps=np.sqrt(0.005))
+    lin = C.Ellip(fs, p=2.0)
+    El = O.Ellip(S, p=2.0)
+    uc = sine_state(S.nodal_coordinates(), 5)
+    fs.dirichlet_commit(uc[fs.nowned:])
+    v = 2 * splitmix64_uniform(19, fs.nowned) - 1
+    vc = np.concatenate([v, np.zeros(fs.nclosure - fs.nowned)])
+    ref_f = E.function(uc)[:fs.nowned]
+    ref_j = E.jacobian_apply(uc, vc)[:fs.nowned]
+    ref_l = El.jacobian_apply(np.zeros(fs.nclosure), vc)[:fs.nowned]
+    for shape in range(4):
+        for mode in ("residual", "jacobian"):
+            op.set_shape(mode, 0, shape)
+            op.set_shape(mode, 1, shape)
+        lin.set_shape("linear", 0, shape)
+        lin.set_shape("linear", 1, shape)
+        f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+        op.function(dev(uc[:fs.nowned]), f)
+        y = torch.zeros_like(f)
+        op.mat_mult(dev(v), y)
+        yl = torch.zeros_like(f)
+        lin.mat_mult(dev(v), yl)
+        y32 = torch.zeros(fs.nowned, dtype=torch.float32, device="cuda")
+        op.mat_mult_f32(dev(v, torch.float32), y32)
+        yl32 = torch.zeros_like(y32)
+        lin.mat_mult_f32(dev(v, torch.float32), yl32)
+        errs = [rel(f.cpu().numpy(), ref_f), rel(y.cpu().numpy(), ref_j), rel(yl.cpu().numpy(), ref_l)]
+        e32 = [rel(y32.cpu().numpy(), ref_j), rel(yl32.cpu().numpy(), ref_l)]
+        print(f"shape {shape}: fp64 {['%.1e' % e for e in errs]} fp32 {['%.1e' % e for e in e32]}")
+        assert max(errs) < TOL64, (shape, errs)
+        assert max(e32) < 1e-4, (shape, e32)
+        assert op.tuning("jacobian", 0)[0] == shape and lin.tuning("linear", 1)[0] == shape
+
+
 def test_atomic_and_colored_scatter_agree(C):
     M, p = (4, 4, 4), 5
     V = block_vertices(M, perturb=0.1, seed=2)

@@ -372,6 +372,18 @@ __device__ __forceinline__ ColGeo<T> column_geometry(const double *X, const doub
   return make_colgeo<T>(Rc, Sc, c2);
 }
 
+// Order of the gather and scatter slots of a workgroup's EPB elements: consecutive slots walk the
+// z lines of consecutive elements, (a, b, element, c), so that when the elements are neighbours along
+// z a wavefront's loads and atomic adds cover long runs of consecutive nodes (atomic throughput
+// depends on it).  Returns the tile index (element * n^3 + (a n + b) n + c) of slot t.
+template <int N, int EPB>
+__device__ __forceinline__ int brick_slot(int t)
+{
+  constexpr int RUN = EPB * N;
+  const int ab = t / RUN, r = t - ab * RUN, le = r / N;
+  return le * N * N * N + ab * N + (r - le * N);
+}
+
 // Threads of a workgroup holding EPB elements: one per line, EPB * n^2.  A workgroup smaller than a
 // wavefront is rounded up to one: its extra lanes repeat lines of the same wavefront, which runs in
 // lockstep, so they load, compute and store the same values at the same time as the lines' owners.

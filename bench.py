@@ -250,6 +250,138 @@ def learned_workload(args):
     print(json.dumps(out))
 
 
+def batch_problem(Me=32, nprob=256):
+    """(mixed-mesh degrees, element vertices, [(M, degree, V)] of the independent problems)."""
+    from dohp_amd.mesh import block_vertices, element_vertices, uniform_integers
+    M = (Me, Me, Me)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=3)
+    deg = uniform_integers(4, Me ** 3, 2, 10)
+    ns, ps = uniform_integers(6, nprob, 4, 16), uniform_integers(7, nprob, 2, 8)
+    probs = []
+    for i in range(nprob):
+        Mi = (int(ns[i]),) * 3
+        probs.append((Mi, int(ps[i]), block_vertices(Mi, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1000 + i)))
+    return deg, element_vertices(M, V), probs
+
+
+def batch_workload(args):
+    import torch
+    import dohp_amd.capi as C
+    from dohp_amd.mesh import splitmix64_uniform
+    t0 = time.perf_counter()
+    Me = args.elems or 32
+    deg, vx, probs = batch_problem(Me)
+    B = C.Batch()
+    seg_mixed = B.add_elements(deg, vx)
+    segs, nbytes = [], {4: 0, 8: 0}
+    for (Mi, pi, Vi) in probs:
+        fs = C.FS(Mi, pi, Vi)
+        segs.append(B.add_space(fs))
+        for s in (4, 8):
+            nbytes[s] += alg_bytes(s, fs.nclosure, fs.nowned, pi + 1, fs.nelem, fs.nvert, False)
+        del fs
+    nexp = int(((deg + 1) ** 3).sum())
+    for s in (4, 8):  # element-local: read x, write y, 8 vertex coordinates per element
+        nbytes[s] += 2 * s * nexp + 24 * 8 * deg.size
+    B.setup()
+    n = B.size
+    x = torch.as_tensor(2 * splitmix64_uniform(9, n) - 1, dtype=torch.float64, device="cuda")
+    x32 = x.float()
+    y, y32 = torch.zeros_like(x), torch.zeros_like(x32)
+    B.mult(x, y)          # the first application of each precision tunes the tile per degree
+    B.mult_f32(x32, y32)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    nel = deg.size + sum(m[0] ** 3 for (m, _, _) in probs)
+    print(f"# setup {t_setup:.1f}s: {nel} elements, {n} packed unknowns ({nexp} element-local)",
+          file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        B.mult_f32(x32, y32, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        B.mult_f32(x32, y32, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_step = e0.elapsed_time(e1) / 1e3 / args.steps
+    t64 = time_fn(torch, lambda: B.mult(x, y, stream), max(3, args.steps // 2))
+    B.mult(x, y, stream)
+    B.mult_f32(x32, y32, stream)
+    torch.cuda.synchronize()
+    err32 = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
+    print(f"# batch fp32 {t_step * 1e6:.1f} us, fp64 {t64 * 1e6:.1f} us, fp32 vs fp64 {err32:.2e}",
+          file=sys.stderr, flush=True)
+    names = ["workgroup", "wavefront", "mfma"]
+    tuning = {}
+    for d in range(1, 11):
+        for fp32 in (0, 1):
+            tile, ms = B.tuning(d, fp32)
+            if tile >= 0:
+                tuning[f"p{d}_{'f32' if fp32 else 'f64'}"] = {"tile": names[tile], "ms": ms}
+    # FP32 with the MFMA contraction tile offered to the tuner from degree 4 up (BF16 operands, FP32
+    # accumulation; 3 = operands split into two BF16 parts), and with it fixed for every degree >= 4
+    mfma, ymf = {}, {}
+    for passes in (3, 1):
+        for forced in (False, True):
+            B.set_mfma(passes)
+            for d in range(4, 11):
+                B.set_tile(d, True, 2 if forced else -1)
+            ym = torch.zeros_like(y32)
+            B.mult_f32(x32, ym, stream)
+            tm = time_fn(torch, lambda: B.mult_f32(x32, ym, stream), max(3, args.steps // 2))
+            B.mult_f32(x32, ym, stream)
+            torch.cuda.synchronize()
+            key = f"bf16x{passes}_{'all' if forced else 'tuned'}"
+            ymf[key] = ym.double().cpu().numpy()
+            mfma[key] = {"us": round(tm * 1e6, 1), "gdofs": round(n / tm / 1e9, 3),
+                         "rel_err_vs_f64": float(torch.linalg.norm(ym.double() - y) / torch.linalg.norm(y)),
+                         "tiles": {f"p{d}": names[B.tuning(d, True)[0]] for d in range(4, 11)
+                                   if B.tuning(d, True)[0] >= 0}}
+            print(f"# batch fp32 {key}: {mfma[key]}", file=sys.stderr, flush=True)
+    B.set_mfma(0)
+    for d in range(4, 11):
+        B.set_tile(d, True, -1)
+    cpu, oracle_err = None, None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        xh, yh, y32h = x.cpu().numpy(), y.cpu().numpy(), y32.double().cpu().numpy()
+        ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        ref = np.zeros(n)
+        tc = time.perf_counter()
+        o, ln = B.segment(seg_mixed)
+        ref[o:o + ln] = O.expanded_poisson_apply(deg, vx, xh[o:o + ln], nthreads=ncores)
+        for s, (Mi, pi, Vi) in zip(segs, probs):
+            o, ln = B.segment(s)
+            S = O.BlockSpace(Mi, pi, Vi)
+            E = O.Ellip(S, p=2.0, nthreads=ncores)
+            ref[o:o + ln] = E.jacobian_apply(np.zeros(S.nclosure),
+                                             np.concatenate([xh[o:o + ln], np.zeros(S.nclosure - S.nowned)]))[:ln]
+        tc = time.perf_counter() - tc
+        nr = np.linalg.norm(ref)
+        oracle_err = {"f64": float(np.linalg.norm(yh - ref) / nr), "f32": float(np.linalg.norm(y32h - ref) / nr)}
+        for key, ym in ymf.items():
+            oracle_err[f"f32_{key}"] = float(np.linalg.norm(ym - ref) / nr)
+        cpu = {"value": round(n / tc / 1e9, 4), "unit": "GDOF/s", "cores": ncores, "kind": "port",
+               "sample": "the whole batch, one oracle application (element loop over OpenMP threads)"}
+    achieved = nbytes[4] / t_step / 1e9
+    out = {"metric": "GDOF/s batched mixed-order apply (FP32)", "value": round(n / t_step / 1e9, 3), "unit": "GDOF/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{Me}^3 mesh of degree U{{2..10}} (element-local) + {len(probs)} Poisson problems "
+                                  "n^3, n U{4..16}, degree U{2..8}, one persistent launch",
+                      "elements": int(nel), "unknowns": int(n), "element_local_unknowns": nexp},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel_us": round(t_step * 1e6, 1),
+                        "algorithmic_bytes": int(nbytes[4])},
+           "cpu_baseline": cpu,
+           "f64": {"us": round(t64 * 1e6, 1), "gdofs": round(n / t64 / 1e9, 3),
+                   "hbm_frac": round(nbytes[8] / t64 / 1e9 / HBM_PEAK_GBS, 4)},
+           "f32_rel_err_vs_f64": err32, "rel_err_vs_oracle": oracle_err, "tuning": tuning, "f32_mfma": mfma}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

@@ -284,14 +284,30 @@ __device__ __forceinline__ void batch_item(const BArgs &g, const int4 it, const 
     backward<N, T, WS>(tab + tab_off(N), A, GX, GY, a, b, vq, d0, d1, d2);
   }
 
-  // scatter: atomics into block spaces, plain stores into element-local slices
+  // scatter: atomics into block spaces, plain stores into element-local slices.  Nodes shared by
+  // consecutive elements of the group (z neighbours in a block space) are folded into the later
+  // element's slot first, and the slots walk (a, b, element, c), so the atomics cover runs of
+  // consecutive, distinct nodes.
   int tl = gt;
   asm volatile("" : "+v"(tl));
+  if constexpr (EW > 1) {
+    if (!excl) {
+      for (int t = tl; t < (EW - 1) * NN; t += GT) {
+        const int le2 = e0 + t / NN, ab2 = t % NN, top = le2 * N3 + ab2 * N + N - 1, bot = top + N3 - N + 1;
+        const int ct = IDX[top];
+        if (ct >= 0 && ct == IDX[bot]) {
+          TA[bot] += TA[top];
+          IDX[top] = -2;
+        }
+      }
+    }
+    tile_sync<WS>();
+  }
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int t = s * GT + tl;
     if (t < EW * N3) {
-      const int ts = e0 * N3 + t, c = IDX[ts];
+      const int ts = e0 * N3 + brick_slot<N, EW>(t), c = IDX[ts];
       if (c >= 0) {
         const T v = TA[ts];
         if (excl) y[c] = v;

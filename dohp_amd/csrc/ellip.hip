@@ -134,11 +134,7 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__
     else                     \
       y[c] += val;           \
   } while (0)
-#define DOHP_ZNEXT(le)                                                                              \
-  (EB[4 * (le) + 3] && EB[4 * (le) + 7] && EB[4 * (le) + 4] == EB[4 * (le)] &&                   \
-   EB[4 * (le) + 5] == EB[4 * (le) + 1] && EB[4 * (le) + 6] == EB[4 * (le) + 2] + bm.p)
 #include "ellip_batch.inc"
-#undef DOHP_ZNEXT
 #undef DOHP_SCATTER
 #undef DOHP_BATCH_HOOK
 }

@@ -102,8 +102,8 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__
   }
   __syncthreads();
   // gather of the first field, coalesced over the element's nodes (tile order = node order); the
-  // Jacobian action prefetches the state into registers behind the direction's contractions
-  T upre[MODE == MODE_JAC ? NS : 1];
+  // Jacobian action keeps its second field in registers until the first one's contractions are done
+  T pre2[MODE == MODE_JAC ? NS : 1];
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int t = s * NT + tid;
@@ -119,8 +119,10 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__
         if (MODE != MODE_FUN && own >= 0) v = xdir[own];
       }
       IDX[t] = own;
-      TA[t] = MODE == MODE_FUN ? uv : v;
-      if (MODE == MODE_JAC) upre[s] = uv;
+      // the FP32 Jacobian action contracts the state first, the FP64 one the direction
+      constexpr bool STATE_FIRST = MODE == MODE_FUN || (MODE == MODE_JAC && sizeof(T) == 4);
+      TA[t] = STATE_FIRST ? uv : v;
+      if (MODE == MODE_JAC) pre2[s] = STATE_FIRST ? v : uv;
     }
   }
   __syncthreads();
@@ -151,7 +153,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
     const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph)
 {
   constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
-  // the Jacobian action prefetches its two fields only in FP32 (in FP64 the registers cost occupancy)
+  // the Jacobian action prefetches its two fields (state, direction) only in FP32 (in FP64 the
+  // registers cost occupancy)
   constexpr bool PREFETCH_VALUES = MODE != MODE_JAC || sizeof(T) == 4;
   constexpr bool PREFETCH_STATE = MODE == MODE_JAC && PREFETCH_VALUES;
   __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
@@ -207,8 +210,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
         int own, lat;
         slot_nodes(bb, s * NT + tl, own, lat);
         pidx[s] = own;
-        pval[s] = MODE == MODE_FUN ? (lat >= 0 ? ustate[lat] : T(0)) : (own >= 0 ? xdir[own] : T(0));
-        if constexpr (PREFETCH_STATE) pst[s] = lat >= 0 ? ustate[lat] : T(0);
+        pval[s] = MODE == MODE_LIN ? (own >= 0 ? xdir[own] : T(0)) : (lat >= 0 ? ustate[lat] : T(0));
+        if constexpr (PREFETCH_STATE) pst[s] = own >= 0 ? xdir[own] : T(0);
       }
     }
   };
@@ -230,10 +233,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
 #pragma unroll
     for (int q = 0; q < NV; q++)
       if (q * NT + tid < EPB * 32) VX[q * NT + tid] = pvx[q];
-    T upre[MODE == MODE_JAC ? NS : 1];
+    T pre2[MODE == MODE_JAC ? NS : 1];
     if constexpr (PREFETCH_STATE) {
 #pragma unroll
-      for (int s = 0; s < NS; s++) upre[s] = pst[s];
+      for (int s = 0; s < NS; s++) pre2[s] = pst[s];
     }
     if constexpr (!PREFETCH_VALUES) {
       int tl = tid;
@@ -244,9 +247,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
         int own, lat;
         slot_nodes(bt, t, own, lat);
         if (t < EPB * N3) {
-          IDX[t] = own;
+          IDX[t] = own;  // FP64 Jacobian action: the direction first, the state in registers
           TA[t] = own >= 0 ? xdir[own] : T(0);
-          upre[s] = lat >= 0 ? ustate[lat] : T(0);
+          pre2[s] = lat >= 0 ? ustate[lat] : T(0);
         }
       }
     }

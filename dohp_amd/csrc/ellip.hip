@@ -58,6 +58,16 @@ __device__ __forceinline__ void node_lattice(int r, int &I, int &J, int &K)
   K += r2 - b * N;
 }
 
+// Jacobian action: which field is contracted first.  The state first, reduced to a symmetric metric
+// per point before the direction's contractions, needs fewer registers (used in FP32, and in FP64 from
+// n = 9, where the direction-first order needs more than 256); the direction first needs fewer
+// operations.
+template <int N, typename T>
+constexpr bool jac_state_first()
+{
+  return sizeof(T) == 4 || N >= 9;
+}
+
 template <int N, typename T, int MODE, int EPB>
 __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__restrict__ elist, long ne, BoxMap bm,
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
@@ -119,8 +129,7 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__
         if (MODE != MODE_FUN && own >= 0) v = xdir[own];
       }
       IDX[t] = own;
-      // the FP32 Jacobian action contracts the state first, the FP64 one the direction
-      constexpr bool STATE_FIRST = MODE == MODE_FUN || (MODE == MODE_JAC && sizeof(T) == 4);
+      constexpr bool STATE_FIRST = MODE == MODE_FUN || (MODE == MODE_JAC && jac_state_first<N, T>());
       TA[t] = STATE_FIRST ? uv : v;
       if (MODE == MODE_JAC) pre2[s] = STATE_FIRST ? v : uv;
     }
@@ -247,9 +256,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
         int own, lat;
         slot_nodes(bt, t, own, lat);
         if (t < EPB * N3) {
-          IDX[t] = own;  // FP64 Jacobian action: the direction first, the state in registers
-          TA[t] = own >= 0 ? xdir[own] : T(0);
-          pre2[s] = lat >= 0 ? ustate[lat] : T(0);
+          IDX[t] = own;
+          const T xv = own >= 0 ? xdir[own] : T(0), uv = lat >= 0 ? ustate[lat] : T(0);
+          TA[t] = jac_state_first<N, T>() ? uv : xv;
+          pre2[s] = jac_state_first<N, T>() ? xv : uv;
         }
       }
     }

@@ -11,7 +11,9 @@
 
 namespace {
 
-enum { MODE_LIN = 0, MODE_FUN = 1, MODE_JAC = 2 };
+// Element kernel modes: linear operator, residual without its forcing term, Jacobian action, and the
+// forcing term alone (assembled once per operator: it does not depend on the state).
+enum { MODE_LIN = 0, MODE_FUN = 1, MODE_JAC = 2, MODE_FRC = 3 };
 
 
 struct Phys {

@@ -69,7 +69,8 @@ constexpr bool jac_state_first()
 }
 
 template <int N, typename T, int MODE, int EPB>
-__global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__restrict__ elist, long ne, BoxMap bm,
+__global__ void __launch_bounds__(wg_threads(N, EPB))
+__attribute__((amdgpu_waves_per_eu(MODE == MODE_FUN && N <= 7 ? 3 : 1))) ellip_kernel(const int *__restrict__ elist, long ne, BoxMap bm,
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
                                                     const T *__restrict__ ustate, const T *__restrict__ xdir,
                                                     T *__restrict__ y, long nowned, int atomic,
@@ -125,8 +126,8 @@ __global__ void __launch_bounds__(wg_threads(N, EPB)) ellip_kernel(const int *__
         int I = EB[4 * l], J = EB[4 * l + 1], K = EB[4 * l + 2];
         node_lattice<N>(t - l * N3, I, J, K);
         own = owned_index(bm, I, J, K);
-        if (MODE != MODE_LIN) uv = ustate[(I * bm.Ly + J) * bm.Lz + K];
-        if (MODE != MODE_FUN && own >= 0) v = xdir[own];
+        if (MODE == MODE_FUN || MODE == MODE_JAC) uv = ustate[(I * bm.Ly + J) * bm.Lz + K];
+        if ((MODE == MODE_LIN || MODE == MODE_JAC) && own >= 0) v = xdir[own];
       }
       IDX[t] = own;
       constexpr bool STATE_FIRST = MODE == MODE_FUN || (MODE == MODE_JAC && jac_state_first<N, T>());
@@ -320,6 +321,8 @@ struct _p_dohpEllip {
   int shape[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // tuned launch shape per [mode][FP32], -1 = not yet
   float shape_ms[3][2][4] = {};
   DevBuf<double> state, kvert, tab64, wphi;  // state in lattice order; wphi: Gauss weights then (1 + z_q)/2
+  DevBuf<double> forcing;                     // assembled forcing term -(f, phi) on the owned nodes
+  bool has_forcing = false;
   DevBuf<float> state32, tab32;
   // host tables
   std::vector<double> B, Dh, w, phi;
@@ -489,6 +492,37 @@ dErr launch(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
   }
 }
 
+// The forcing term of the residual (exact solutions 0..4), assembled on first use.
+dErr assemble_forcing(_p_dohpEllip *op, hipStream_t st)
+{
+  const dohpFS fs = op->fs;
+  DOHP_CALL(op->forcing.alloc(fs->nowned));
+  hipLaunchKernelGGL(zero_kernel<double>, dim3((fs->nowned + 255) / 256), dim3(256), 0, st, op->forcing.ptr,
+                     fs->nowned);
+  const Phys ph = make_phys(op);
+  switch (fs->n) {
+#define DOHP_CASE(NN)                                                                                  \
+  case NN:                                                                                             \
+    launch_shape<NN, double, MODE_FRC, epb_wide(NN)>(op, nullptr, nullptr, op->forcing.ptr, st, ph); \
+    break;
+    DOHP_CASE(2)
+    DOHP_CASE(3)
+    DOHP_CASE(4)
+    DOHP_CASE(5)
+    DOHP_CASE(6)
+    DOHP_CASE(7)
+    DOHP_CASE(8)
+    DOHP_CASE(9)
+    DOHP_CASE(10)
+    DOHP_CASE(11)
+#undef DOHP_CASE
+  default: set_error("fused operator supports degree 1..10"); return dohpERR_ARG;
+  }
+  DOHP_HIP(hipGetLastError());
+  op->has_forcing = true;
+  return 0;
+}
+
 template <typename T>
 dErr zero(T *y, long n, hipStream_t st)
 {
@@ -610,7 +644,12 @@ extern "C" dErr dohpEllipFunction(dohpEllip op, const double *u, double *f, void
   hipLaunchKernelGGL(lattice_state_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
                      fs->d_lat2clo.ptr, fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
   op->has_state = true;
-  DOHP_CALL(zero(f, fs->nowned, st));
+  if (op->prm.exact >= 0) {  // y = forcing term, then the state's terms are added
+    if (!op->has_forcing) DOHP_CALL(assemble_forcing(op, st));
+    DOHP_HIP(hipMemcpyAsync(f, op->forcing.ptr, sizeof(double) * fs->nowned, hipMemcpyDeviceToDevice, st));
+  } else {
+    DOHP_CALL(zero(f, fs->nowned, st));
+  }
   return launch<double, MODE_FUN>(op, op->state.ptr, nullptr, f, st);
 }
 

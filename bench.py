@@ -8,9 +8,15 @@ p = 1.5, eps^2 = 0.005 has eta = 2^(1/4) k, so the operator is ellip's up to tha
 
 A step is one Jacobian-vector product y = J(u0) v (the MATSHELL MatMult a Krylov solver calls)
 in FP64 with v ~ U[-1,1] (seeded), inputs resident in HBM.  value = N_int / t_step in GDOF/s.
-The residual F(u0), the FP32 Jacobian action and the linear (Poisson) apply are timed too and
-reported under "variants".  roofline.achieved divides the algorithmic bytes of one J v by the
-element kernel's own duration (y += J v, timed with HIP events on its stream); roofline.traffic
+As in the reference, the Jacobian reads per-point data that the linearization leaves behind (here
+one symmetric 3x3 tensor per Gauss point, 48 bytes, formed by the first application after the
+residual F(u0) and reused by every later one); the time to form it is reported beside the step
+("quadrature_data_f64"), and the same applications recomputing everything from the state at every
+call ("*_recomputed").  The residual F(u0), the FP32 Jacobian action and the linear (Poisson) apply
+(recomputed by default, from stored data as "linear_*_stored") are timed too and reported under
+"variants".  roofline.achieved divides the algorithmic bytes of one
+J v (x read, y written, the quadrature data read) by the element kernel's own duration (y += J v,
+timed with HIP events on its stream); roofline.traffic
 is the HBM traffic of one launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, read from the
 profile summary committed under profiles/ when it covers this workload.
 
@@ -66,9 +72,15 @@ def sine_state(X, seed):
     return u
 
 
-def alg_bytes(s, n_tot, n_int, n, nelem, nvert, jv):
-    """Minimum unique HBM traffic of one apply (geometry recomputed from Q1 vertices)."""
-    b = s * n_tot + s * n_int + 4 * n ** 3 * nelem + 24 * nvert + 32 * nelem
+def alg_bytes(s, n_tot, n_int, n, nelem, nvert, jv, nq=0, maps=False):
+    """Minimum unique HBM traffic of one apply: x read and y written once (s bytes per value), plus
+    either the stored quadrature data (nq values per point, n^3 points per element) or the Q1 vertex
+    coordinates for geometry recomputed in the kernel and, for the recomputed Jacobian, the state on
+    the closure.  maps: the kernel also reads an element -> node index table (4 bytes per node) and an
+    element -> vertex table (the block-space kernels compute both from the block numbering)."""
+    if nq:
+        return 2 * s * n_int + nq * s * n ** 3 * nelem
+    b = 2 * s * n_int + 24 * nvert + ((4 * n ** 3 + 32) * nelem if maps else 0)
     return b + (s * n_tot if jv else 0)
 
 
@@ -121,6 +133,8 @@ def measured_traffic(workload):
 
 
 def copy_probe():
+    """Plain 1 GiB device copies (the calibration kernel of the HBM counters), timed, and the practical
+    streaming rates of this GPU: copy (read + write) and a read-only reduction over 4 GiB."""
     import torch
     n = 1 << 27
     a = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -128,7 +142,11 @@ def copy_probe():
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 3}))
+    t_copy = time_fn(torch, lambda: b.copy_(a), 10)
+    big = torch.ones(4 * n, dtype=torch.float64, device="cuda")
+    t_read = time_fn(torch, lambda: big.sum(), 10)
+    print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 3,
+                      "copy_GBs": round(16 * n / t_copy / 1e9, 1), "read_GBs": round(32 * n / t_read / 1e9, 1)}))
 
 
 def solve_workload(args):
@@ -278,7 +296,7 @@ def batch_workload(args):
         fs = C.FS(Mi, pi, Vi)
         segs.append(B.add_space(fs))
         for s in (4, 8):
-            nbytes[s] += alg_bytes(s, fs.nclosure, fs.nowned, pi + 1, fs.nelem, fs.nvert, False)
+            nbytes[s] += alg_bytes(s, fs.nclosure, fs.nowned, pi + 1, fs.nelem, fs.nvert, False, maps=True)
         del fs
     nexp = int(((deg + 1) ** 3).sum())
     for s in (4, 8):  # element-local: read x, write y, 8 vertex coordinates per element
@@ -434,6 +452,7 @@ def main():
     y32 = torch.zeros_like(v32)
     stream = torch.cuda.current_stream()
     op.function(u, f, stream)
+    op.mat_mult(v, y, stream)  # the linearization's quadrature data is formed here (reported below)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f}s: {fs.nelem} elements p={p}, N_int={fs.nowned}, N_tot={fs.nclosure}",
@@ -459,26 +478,53 @@ def main():
 
     variants = {}
     reps = max(3, args.steps // 2)
-    for name, fn, s, jv in [("residual_f64", lambda: op.function(u, f, stream), 8, True),
-                            ("jv_f32", lambda: op.mat_mult_f32(v32, y32, stream), 4, True),
-                            ("linear_f64", lambda: lin.mat_mult(v, y, stream), 8, False),
-                            ("linear_f32", lambda: lin.mat_mult_f32(v32, y32, stream), 4, False)]:
+
+    def timed(name, fn, s, jv, nq):
         t = time_fn(torch, fn, reps)
-        b = alg_bytes(s, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, jv)
+        b = alg_bytes(s, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, jv, nq=nq)
         variants[name] = {"us": round(t * 1e6, 1), "gdofs": round(fs.nowned / t / 1e9, 2),
                           "hbm_frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}
         print(f"# {name} {t * 1e6:.1f} us", file=sys.stderr, flush=True)
+        return t
+
+    t_res = timed("residual_f64", lambda: op.function(u, f, stream), 8, True, 0)
+    # forming the quadrature data: the first application after a residual pays it
+    t_pair = time_fn(torch, lambda: (op.function(u, f, stream), op.mat_mult(v, y, stream)), reps)
+    variants["quadrature_data_f64"] = {"us": round((t_pair - t_res - t_step) * 1e6, 1),
+                                       "bytes": 6 * 8 * n ** 3 * fs.nelem}
+    print(f"# quadrature data {(t_pair - t_res - t_step) * 1e6:.1f} us", file=sys.stderr, flush=True)
+    timed("jv_f32", lambda: op.mat_mult_f32(v32, y32, stream), 4, True, 6)
+    timed("linear_f64", lambda: lin.mat_mult(v, y, stream), 8, False, 0)
+    timed("linear_f32", lambda: lin.mat_mult_f32(v32, y32, stream), 4, False, 0)
     # FP32 error against the FP64 apply (same inputs)
     op.mat_mult(v, y, stream)
     op.mat_mult_f32(v32, y32, stream)
     torch.cuda.synchronize()
     variants["jv_f32"]["rel_err_vs_f64"] = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
+    tuning = {f"stored_{'f32' if f else 'f64'}": op.tuning("stored", f) for f in (0, 1)}
+    tuning["residual_f64"] = op.tuning("residual", 0)
+    tuning.update({f"linear_{'f32' if f else 'f64'}": lin.tuning("linear", f) for f in (0, 1)})
+    # the linear operator from stored data (by default it recomputes: faster at this order)
+    lin.set_storage(1)
+    timed("linear_f64_stored", lambda: lin.mat_mult(v, y, stream), 8, False, 6)
+    timed("linear_f32_stored", lambda: lin.mat_mult_f32(v32, y32, stream), 4, False, 6)
+    tuning.update({f"linear_stored_{'f32' if f else 'f64'}": lin.tuning("stored", f) for f in (0, 1)})
+    lin.set_storage(2)
+    # the Jacobian recomputing the state's gradient and the geometry at every application (no storage)
+    op.set_storage(0)
+    yr = torch.zeros_like(y)
+    op.mat_mult(v, yr, stream)
+    torch.cuda.synchronize()
+    variants["stored_vs_recomputed_rel_diff"] = float(torch.linalg.norm(yr - y) / torch.linalg.norm(y))
+    timed("jv_f64_recomputed", lambda: op.mat_mult(v, y, stream), 8, True, 0)
+    timed("jv_f32_recomputed", lambda: op.mat_mult_f32(v32, y32, stream), 4, True, 0)
+    tuning.update({f"jacobian_{'f32' if f else 'f64'}": op.tuning("jacobian", f) for f in (0, 1)})
+    variants["tuning"] = tuning
+    op.set_storage(2)
 
-    variants["tuning"] = {f"{m}_{'f32' if f else 'f64'}": (lin if m == "linear" else op).tuning(m, f)
-                          for m, f in (("jacobian", 0), ("jacobian", 1), ("residual", 0), ("linear", 0), ("linear", 1))}
-    B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True)
+    B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True, nq=6)
     achieved = B / (t_kernel or t_step) / 1e9
-    workload = f"nonlinear diffusion J(u0)v, {Me}^3 hex, p={p}, FP64, perturbed mesh"
+    workload = f"nonlinear diffusion J(u0)v from stored quadrature data, {Me}^3 hex, p={p}, FP64, perturbed mesh"
     cpu = None
     if not args.no_cpu:
         ncores = os.cpu_count() or 1
@@ -502,7 +548,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload, "elements": fs.nelem, "degree": p, "n_int": fs.nowned,
-                   "n_tot": fs.nclosure, "scatter": "colored" if args.scatter == 0 else "atomic"},
+                   "n_tot": fs.nclosure, "scatter": "colored" if args.scatter == 0 else "atomic",
+                   "jacobian_data": "stored per Gauss point (6 FP64 values)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload),
                      "kernel_us": round((t_kernel or t_step) * 1e6, 1), "algorithmic_bytes": int(B)},

@@ -70,6 +70,7 @@ _proto("dohpEllipGetTuning", _vp, ctypes.c_int, ctypes.c_int, _vp, _vp)
 _proto("dohpEllipSetShape", _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int)
 _proto("dohpEllipAssembleQ1", _vp, _vp, ctypes.POINTER(_vp), _vp)
 _proto("dohpEllipSetQ1Mode", _vp, ctypes.c_int)
+_proto("dohpEllipSetStorage", _vp, ctypes.c_int)
 _proto("dohpMatGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long))
 _proto("dohpMatGetCSR", _vp, _ip, _ip, _dp)
 _proto("dohpMatMult", _vp, _vp, _vp, _vp)
@@ -379,17 +380,27 @@ class Ellip:
             k, kp = _np(kvert)
             check(_lib.dohpEllipSetCoefficient(self.h, kp))
 
+    _TUNE_MODES = {"linear": 0, "residual": 1, "jacobian": 2, "stored": 3}
+
     def tuning(self, mode, fp32=False):
         """(chosen shape, ms of [four-wavefront, one-wavefront, persistent one-wavefront,
-        persistent four-wavefront])."""
+        persistent four-wavefront]) of a mode: "linear" and "jacobian" recompute the geometry (storage 0),
+        "stored" applies either operator from its quadrature data (storage 1)."""
         sh, ms = ctypes.c_int(), (ctypes.c_float * 4)()
-        check(_lib.dohpEllipGetTuning(self.h, {"linear": 0, "residual": 1, "jacobian": 2}[mode], int(fp32),
-                                      ctypes.byref(sh), ms))
+        check(_lib.dohpEllipGetTuning(self.h, self._TUNE_MODES[mode], int(fp32), ctypes.byref(sh), ms))
         return sh.value, [round(v, 4) for v in ms]
 
     def set_shape(self, mode, fp32, shape):
-        """Fix the launch shape of a mode ("linear", "residual", "jacobian") and precision (-1 = tune)."""
-        check(_lib.dohpEllipSetShape(self.h, {"linear": 0, "residual": 1, "jacobian": 2}[mode], int(fp32), int(shape)))
+        """Fix the launch shape of a mode ("linear", "residual", "jacobian", "stored") and precision
+        (-1 = tune)."""
+        check(_lib.dohpEllipSetShape(self.h, self._TUNE_MODES[mode], int(fp32), int(shape)))
+
+    def set_storage(self, mode):
+        """1: applications read per-point quadrature data formed once per linearization (48 bytes
+        per point in FP64, 56 with lambda; half again for FP32 applications); 0: recompute from the
+        state and the geometry at every application, no storage; 2 (default): 1 for the nonlinear
+        Jacobian, 0 for the linear operator."""
+        check(_lib.dohpEllipSetStorage(self.h, int(mode)))
 
     def set_scatter(self, mode):
         check(_lib.dohpEllipSetScatter(self.h, int(mode)))

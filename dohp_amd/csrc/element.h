@@ -11,9 +11,12 @@
 
 namespace {
 
-// Element kernel modes: linear operator, residual without its forcing term, Jacobian action, and the
-// forcing term alone (assembled once per operator: it does not depend on the state).
-enum { MODE_LIN = 0, MODE_FUN = 1, MODE_JAC = 2, MODE_FRC = 3 };
+// Element kernel modes: linear operator, residual without its forcing term, Jacobian action, the
+// forcing term alone (assembled once per operator: it does not depend on the state), the per-point
+// quadrature data of the Jacobian (MODE_QD, written once per linearization) and the Jacobian action
+// (or linear operator) that reads it (MODE_JQD).
+enum { MODE_LIN = 0, MODE_FUN = 1, MODE_JAC = 2, MODE_FRC = 3, MODE_QD = 4, MODE_JQD = 5 };
+constexpr int kModes = 6;
 
 
 struct Phys {

@@ -92,11 +92,13 @@ __attribute__((amdgpu_waves_per_eu(MODE == MODE_FUN && N <= 7 ? 3 : 1))) ellip_k
       const int e = elist ? elist[es] : (int)es;
       int ex, ey, ez;
       elem_coords(bm, e, ex, ey, ez);
-      const long vi = vertex_index(bm, ex, ey, ez, vt);
-      dst[0] = vxyz[3 * vi];
-      dst[1] = vxyz[3 * vi + 1];
-      dst[2] = vxyz[3 * vi + 2];
-      dst[3] = ph.has_k ? kv[vi] : 1.0;
+      if (MODE != MODE_JQD) {  // (the stored quadrature data carries the geometry)
+        const long vi = vertex_index(bm, ex, ey, ez, vt);
+        dst[0] = vxyz[3 * vi];
+        dst[1] = vxyz[3 * vi + 1];
+        dst[2] = vxyz[3 * vi + 2];
+        dst[3] = ph.has_k ? kv[vi] : 1.0;
+      }
       if (vt == 0) {
         EB[4 * l] = ex * bm.p;
         EB[4 * l + 1] = ey * bm.p;
@@ -113,31 +115,55 @@ __attribute__((amdgpu_waves_per_eu(MODE == MODE_FUN && N <= 7 ? 3 : 1))) ellip_k
   }
   __syncthreads();
   // gather of the first field, coalesced over the element's nodes (tile order = node order); the
-  // Jacobian action keeps its second field in registers until the first one's contractions are done
+  // Jacobian action keeps its second field in registers until the first one's contractions are done.
+  // Every slot's indices first, then every load (slots without a value read entry 0 and drop it: no
+  // branch separates the loads, so they are in flight together), then the tile.
   T pre2[MODE == MODE_JAC ? NS : 1];
+  {
+    constexpr bool NEED_U = MODE == MODE_FUN || MODE == MODE_JAC || MODE == MODE_QD;
+    constexpr bool NEED_X = MODE == MODE_LIN || MODE == MODE_JAC || MODE == MODE_JQD;
+    constexpr bool STATE_FIRST =
+        MODE == MODE_FUN || MODE == MODE_QD || (MODE == MODE_JAC && jac_state_first<N, T>());
+    int own[NS], lat[NS];
 #pragma unroll
-  for (int s = 0; s < NS; s++) {
-    const int t = s * NT + tid;
-    if (t < EPB * N3) {
-      const int l = t / N3;
-      int own = -1;
-      T v = 0, uv = 0;
-      if (EB[4 * l + 3]) {
-        int I = EB[4 * l], J = EB[4 * l + 1], K = EB[4 * l + 2];
-        node_lattice<N>(t - l * N3, I, J, K);
-        own = owned_index(bm, I, J, K);
-        if (MODE == MODE_FUN || MODE == MODE_JAC) uv = ustate[(I * bm.Ly + J) * bm.Lz + K];
-        if ((MODE == MODE_LIN || MODE == MODE_JAC) && own >= 0) v = xdir[own];
+    for (int s = 0; s < NS; s++) {
+      const int t = s * NT + tid;
+      const bool in = t < EPB * N3;
+      const int l = in ? t / N3 : 0;
+      int I = EB[4 * l], J = EB[4 * l + 1], K = EB[4 * l + 2];
+      node_lattice<N>(t - l * N3, I, J, K);
+      const bool live = in && EB[4 * l + 3];
+      own[s] = live ? owned_index(bm, I, J, K) : -1;
+      lat[s] = live ? (I * bm.Ly + J) * bm.Lz + K : -1;
+    }
+    T xv[NS], uv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) xv[s] = uv[s] = T(0);
+    if (NEED_X && nowned > 0) {
+#pragma unroll
+      for (int s = 0; s < NS; s++) xv[s] = xdir[own[s] > 0 ? own[s] : 0];
+    }
+    // (the quadrature data of a linear operator is formed without a state)
+    if (NEED_U && ustate) {
+#pragma unroll
+      for (int s = 0; s < NS; s++) uv[s] = ustate[lat[s] > 0 ? lat[s] : 0];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int t = s * NT + tid;
+      if (t < EPB * N3) {
+        const T v = own[s] >= 0 ? xv[s] : T(0), u = lat[s] >= 0 ? uv[s] : T(0);
+        IDX[t] = own[s];
+        TA[t] = STATE_FIRST ? u : v;
+        if (MODE == MODE_JAC) pre2[s] = STATE_FIRST ? v : u;
       }
-      IDX[t] = own;
-      constexpr bool STATE_FIRST = MODE == MODE_FUN || (MODE == MODE_JAC && jac_state_first<N, T>());
-      TA[t] = STATE_FIRST ? uv : v;
-      if (MODE == MODE_JAC) pre2[s] = STATE_FIRST ? v : uv;
     }
   }
   __syncthreads();
   int tid_body = tid;  // derived values are formed here, not hoisted above the gather
   asm volatile("" : "+v"(tid_body));
+#define DOHP_ELEM(l) \
+  (lb * EPB + (l) < ne ? (elist ? (long)elist[lb * EPB + (l)] : lb * EPB + (l)) : -1L)
 #define DOHP_BATCH_HOOK() ((void)0)
 #define DOHP_SCATTER(c, val) \
   do {                       \
@@ -149,6 +175,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == MODE_FUN && N <= 7 ? 3 : 1))) ellip_k
 #include "ellip_batch.inc"
 #undef DOHP_SCATTER
 #undef DOHP_BATCH_HOOK
+#undef DOHP_ELEM
 }
 
 // Persistent form of the kernel (one wavefront per workgroup, atomic scatter): each workgroup
@@ -166,6 +193,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
   // the Jacobian action prefetches its two fields (state, direction) only in FP32 (in FP64 the
   // registers cost occupancy)
   constexpr bool PREFETCH_VALUES = MODE != MODE_JAC || sizeof(T) == 4;
+  constexpr bool GATHER_X = MODE == MODE_LIN || MODE == MODE_JQD;  // else the state
   constexpr bool PREFETCH_STATE = MODE == MODE_JAC && PREFETCH_VALUES;
   __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
   __shared__ int IDX[EPB * N3];
@@ -200,7 +228,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
       if (t < EPB * 32) {
         const int l = t >> 5, vt = (t >> 2) & 7, d = t & 3;
         const long es = bb * EPB + l;
-        if (es < ne) {
+        if (MODE == MODE_JQD) {  // the stored quadrature data carries the geometry
+          pvx[q] = 0.0;
+        } else if (es < ne) {
           int ex, ey, ez;
           elem_coords(bm, (int)es, ex, ey, ez);
           const long vi = vertex_index(bm, ex, ey, ez, vt);
@@ -220,7 +250,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
         int own, lat;
         slot_nodes(bb, s * NT + tl, own, lat);
         pidx[s] = own;
-        pval[s] = MODE == MODE_LIN ? (own >= 0 ? xdir[own] : T(0)) : (lat >= 0 ? ustate[lat] : T(0));
+        pval[s] = GATHER_X ? (own >= 0 ? xdir[own] : T(0)) : (lat >= 0 && ustate ? ustate[lat] : T(0));
         if constexpr (PREFETCH_STATE) pst[s] = own >= 0 ? xdir[own] : T(0);
       }
     }
@@ -272,6 +302,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
     int tid_body = tid;
     asm volatile("" : "+v"(tid_body));
     const int atomic = 1;
+#define DOHP_ELEM(l) (bt * EPB + (l) < ne ? bt * EPB + (l) : -1L)
 #define DOHP_BATCH_HOOK() \
   do {                    \
     if (nxt < hi) load_val(nxt); \
@@ -286,6 +317,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
 #include "ellip_batch.inc"
 #undef DOHP_SCATTER
 #undef DOHP_BATCH_HOOK
+#undef DOHP_ELEM
     __syncthreads();  // the next batch overwrites the tiles
   }
 }
@@ -318,8 +350,15 @@ struct _p_dohpEllip {
   int scatter = 1;  // 0 = colours (deterministic), 1 = atomics
   int q1mode = 0;   // embedded-Q1 assembly: 0 = the reference's arithmetic, 1 = corrected sub-cell derivatives
   bool has_state = false;
-  int shape[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // tuned launch shape per [mode][FP32], -1 = not yet
-  float shape_ms[3][2][4] = {};
+  int shape[kModes][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};  // tuned per [mode][FP32]
+  float shape_ms[kModes][2][4] = {};
+  // Jacobian (or linear operator) quadrature data, 6 values per point (7 with the Bratu term), formed
+  // on the first application after each residual (the linear operator's once).  storage: 0 = never,
+  // 1 = always, 2 = for the nonlinear Jacobian only (the faster choice for each, measured at n = 7)
+  int storage = 2;
+  DevBuf<double> qd;
+  DevBuf<float> qd32;
+  bool qd_ok = false, qd32_ok = false;
   DevBuf<double> state, kvert, tab64, wphi;  // state in lattice order; wphi: Gauss weights then (1 + z_q)/2
   DevBuf<double> forcing;                     // assembled forcing term -(f, phi) on the owned nodes
   bool has_forcing = false;
@@ -492,18 +531,18 @@ dErr launch(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
   }
 }
 
-// The forcing term of the residual (exact solutions 0..4), assembled on first use.
-dErr assemble_forcing(_p_dohpEllip *op, hipStream_t st)
+bool is_linear(const _p_dohpEllip *op) { return op->prm.p == 2.0 && op->prm.lambda == 0.0; }
+
+// One launch of the four-wavefront shape, outside the tuner (modes whose output is not a vector of
+// owned nodes: the forcing term's is, but it runs once).
+template <int MODE>
+dErr launch_fixed(const _p_dohpEllip *op, const double *ust, double *y, hipStream_t st)
 {
-  const dohpFS fs = op->fs;
-  DOHP_CALL(op->forcing.alloc(fs->nowned));
-  hipLaunchKernelGGL(zero_kernel<double>, dim3((fs->nowned + 255) / 256), dim3(256), 0, st, op->forcing.ptr,
-                     fs->nowned);
   const Phys ph = make_phys(op);
-  switch (fs->n) {
-#define DOHP_CASE(NN)                                                                                  \
-  case NN:                                                                                             \
-    launch_shape<NN, double, MODE_FRC, epb_wide(NN)>(op, nullptr, nullptr, op->forcing.ptr, st, ph); \
+  switch (op->fs->n) {
+#define DOHP_CASE(NN)                                                                  \
+  case NN:                                                                             \
+    launch_shape<NN, double, MODE, epb_wide(NN)>(op, ust, nullptr, y, st, ph);       \
     break;
     DOHP_CASE(2)
     DOHP_CASE(3)
@@ -519,6 +558,56 @@ dErr assemble_forcing(_p_dohpEllip *op, hipStream_t st)
   default: set_error("fused operator supports degree 1..10"); return dohpERR_ARG;
   }
   DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+// Quadrature data of the current linearization (of the mesh and k alone for the linear operator),
+// formed once and reused by every application until the state or the coefficient changes: the
+// stash that the reference's residual leaves for its Jacobian action (ellip.c:311, 333-342, 460-467),
+// here reduced with the geometry to one symmetric tensor per point.
+dErr ensure_qd(_p_dohpEllip *op, hipStream_t st)
+{
+  if (op->qd_ok) return 0;
+  const dohpFS fs = op->fs;
+  // one element's worth of slack at the end: the kernels may load the Bratu component of the last
+  // element's data ahead of the (uniform) test that it exists
+  const size_t n = ((size_t)fs->nelem * (op->prm.lambda != 0 ? 7 : 6) + 1) * fs->n * fs->n * fs->n;
+  if (op->qd.n != n) DOHP_CALL(op->qd.alloc(n));
+  DOHP_CALL(launch_fixed<MODE_QD>(op, is_linear(op) ? nullptr : op->state.ptr, op->qd.ptr, st));
+  op->qd_ok = true;
+  op->qd32_ok = false;
+  return 0;
+}
+
+__global__ void round_f32_kernel(const double *__restrict__ a, float *__restrict__ b, long n)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+dErr ensure_qd32(_p_dohpEllip *op, hipStream_t st)
+{
+  if (op->qd32_ok && op->qd_ok) return 0;
+  DOHP_CALL(ensure_qd(op, st));
+  const long n = (long)op->qd.n;
+  if (op->qd32.n != op->qd.n) DOHP_CALL(op->qd32.alloc(op->qd.n));
+  hipLaunchKernelGGL(round_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, op->qd.ptr, op->qd32.ptr,
+                     n);
+  DOHP_HIP(hipGetLastError());
+  op->qd32_ok = true;
+  return 0;
+}
+
+void drop_qd(_p_dohpEllip *op) { op->qd_ok = op->qd32_ok = false; }
+
+// The forcing term of the residual (exact solutions 0..4), assembled on first use.
+dErr assemble_forcing(_p_dohpEllip *op, hipStream_t st)
+{
+  const dohpFS fs = op->fs;
+  DOHP_CALL(op->forcing.alloc(fs->nowned));
+  hipLaunchKernelGGL(zero_kernel<double>, dim3((fs->nowned + 255) / 256), dim3(256), 0, st, op->forcing.ptr,
+                     fs->nowned);
+  DOHP_CALL(launch_fixed<MODE_FRC>(op, nullptr, op->forcing.ptr, st));
   op->has_forcing = true;
   return 0;
 }
@@ -608,27 +697,32 @@ extern "C" dErr dohpEllipSetScatter(dohpEllip op, int mode)
   return 0;
 }
 
+// tuning modes of the C API: 0 linear, 1 residual, 2 Jacobian recomputed, 3 apply from quadrature data
+static const int kApiMode[4] = {MODE_LIN, MODE_FUN, MODE_JAC, MODE_JQD};
+
 extern "C" dErr dohpEllipGetTuning(dohpEllip op, int mode, int fp32, int *shape, float *ms)
 {
-  DOHP_CHECK(op && mode >= 0 && mode < 3 && (fp32 == 0 || fp32 == 1), dohpERR_ARG, "bad argument");
-  if (shape) *shape = op->shape[mode][fp32];
+  DOHP_CHECK(op && mode >= 0 && mode < 4 && (fp32 == 0 || fp32 == 1), dohpERR_ARG, "bad argument");
+  const int m = kApiMode[mode];
+  if (shape) *shape = op->shape[m][fp32];
   if (ms)
-    for (int v = 0; v < 4; v++) ms[v] = op->shape_ms[mode][fp32][v];
+    for (int v = 0; v < 4; v++) ms[v] = op->shape_ms[m][fp32][v];
   return 0;
 }
 
 extern "C" dErr dohpEllipSetShape(dohpEllip op, int mode, int fp32, int shape)
 {
-  DOHP_CHECK(op && mode >= 0 && mode < 3 && (fp32 == 0 || fp32 == 1) && shape >= -1 && shape < 4, dohpERR_ARG,
+  DOHP_CHECK(op && mode >= 0 && mode < 4 && (fp32 == 0 || fp32 == 1) && shape >= -1 && shape < 4, dohpERR_ARG,
              "bad argument");
   DOHP_CHECK(shape < 2 || op->scatter == 1, dohpERR_ARG, "the persistent shapes need the atomic scatter");
-  op->shape[mode][fp32] = shape;
+  op->shape[kApiMode[mode]][fp32] = shape;
   return 0;
 }
 
 extern "C" dErr dohpEllipSetCoefficient(dohpEllip op, const double *kvert)
 {
   DOHP_CHECK(op, dohpERR_ARG, "null operator");
+  drop_qd(op);
   if (!kvert) {
     op->kvert.release();
     return 0;
@@ -644,6 +738,7 @@ extern "C" dErr dohpEllipFunction(dohpEllip op, const double *u, double *f, void
   hipLaunchKernelGGL(lattice_state_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
                      fs->d_lat2clo.ptr, fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
   op->has_state = true;
+  drop_qd(op);
   if (op->prm.exact >= 0) {  // y = forcing term, then the state's terms are added
     if (!op->has_forcing) DOHP_CALL(assemble_forcing(op, st));
     DOHP_HIP(hipMemcpyAsync(f, op->forcing.ptr, sizeof(double) * fs->nowned, hipMemcpyDeviceToDevice, st));
@@ -659,6 +754,7 @@ extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hi
   hipLaunchKernelGGL(lattice_state_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
                      fs->d_lat2clo.ptr, fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
   op->has_state = true;
+  drop_qd(op);
   DOHP_HIP(hipGetLastError());
   return 0;
 }
@@ -682,26 +778,49 @@ extern "C" dErr dohpEllipSetQ1Mode(dohpEllip op, int mode)
   return 0;
 }
 
-static bool is_linear(const _p_dohpEllip *op) { return op->prm.p == 2.0 && op->prm.lambda == 0.0; }
+namespace {
+
+// y += J x (A x for the linear operator): from the stored quadrature data, or recomputed from the state
+// and the geometry at every application (storage 0)
+bool stored(const _p_dohpEllip *op) { return op->storage == 1 || (op->storage == 2 && !is_linear(op)); }
+
+dErr apply_add(_p_dohpEllip *op, const double *x, double *y, hipStream_t st)
+{
+  if (!is_linear(op)) DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
+  if (stored(op)) {
+    DOHP_CALL(ensure_qd(op, st));
+    return launch<double, MODE_JQD>(op, op->qd.ptr, x, y, st);
+  }
+  if (is_linear(op)) return launch<double, MODE_LIN>(op, nullptr, x, y, st);
+  return launch<double, MODE_JAC>(op, op->state.ptr, x, y, st);
+}
+
+dErr apply_add(_p_dohpEllip *op, const float *x, float *y, hipStream_t st)
+{
+  if (!is_linear(op)) DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
+  if (stored(op)) {
+    DOHP_CALL(ensure_qd32(op, st));
+    return launch<float, MODE_JQD>(op, op->qd32.ptr, x, y, st);
+  }
+  if (is_linear(op)) return launch<float, MODE_LIN>(op, nullptr, x, y, st);
+  return launch<float, MODE_JAC>(op, op->state32.ptr, x, y, st);
+}
+
+}  // namespace
 
 extern "C" dErr dohpEllipShellMatMult(dohpEllip op, const double *x, double *y, void *stream)
 {
   DOHP_CHECK(op && x && y, dohpERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
   DOHP_CALL(zero(y, op->fs->nowned, st));
-  if (is_linear(op)) return launch<double, MODE_LIN>(op, nullptr, x, y, st);
-  DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
-  return launch<double, MODE_JAC>(op, op->state.ptr, x, y, st);
+  return apply_add(op, x, y, st);
 }
 
 extern "C" dErr dohpEllipShellMatMultAdd(dohpEllip op, const double *x, double *y, void *stream)
 {
   DOHP_CHECK(op && x && y, dohpERR_ARG, "null argument");
   DOHP_CHECK(op->scatter == 1, dohpERR_ARG, "MatMultAdd accumulates with the atomic scatter only");
-  hipStream_t st = (hipStream_t)stream;
-  if (is_linear(op)) return launch<double, MODE_LIN>(op, nullptr, x, y, st);
-  DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
-  return launch<double, MODE_JAC>(op, op->state.ptr, x, y, st);
+  return apply_add(op, x, y, (hipStream_t)stream);
 }
 
 extern "C" dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x, float *y, void *stream)
@@ -709,7 +828,18 @@ extern "C" dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x, float *y,
   DOHP_CHECK(op && x && y, dohpERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
   DOHP_CALL(zero(y, op->fs->nowned, st));
-  if (is_linear(op)) return launch<float, MODE_LIN>(op, nullptr, x, y, st);
-  DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
-  return launch<float, MODE_JAC>(op, op->state32.ptr, x, y, st);
+  return apply_add(op, x, y, st);
+}
+
+extern "C" dErr dohpEllipSetStorage(dohpEllip op, int mode)
+{
+  DOHP_CHECK(op && mode >= 0 && mode <= 2, dohpERR_ARG,
+             "storage must be 0 (recompute), 1 (quadrature data) or 2 (quadrature data for the nonlinear Jacobian)");
+  op->storage = mode;
+  if (!stored(op)) {  // the memory is returned
+    drop_qd(op);
+    op->qd.release();
+    op->qd32.release();
+  }
+  return 0;
 }

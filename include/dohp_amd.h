@@ -113,8 +113,20 @@ dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x_dev, float *y_dev, vo
 /* Scatter strategy of the element kernels: 0 = eight-colour deterministic (bitwise reproducible),
  * 1 = hardware atomics (default; summation order varies at rounding level). */
 dErr dohpEllipSetScatter(dohpEllip op, int mode);
+/* How y = J x (and the linear operator) are applied.  1: the first application after each residual
+ * (after creation, or after dohpEllipSetCoefficient, for the linear operator) stores per quadrature
+ * point the symmetric 3x3 tensor G with d_m = sum_n G_mn dx/dX_n (weights, inverse Jacobians, eta and
+ * the rank-one term folded in) and, when lambda != 0, the Bratu factor: 48 bytes per point in FP64
+ * (56 with lambda), plus half that once an FP32 application asks for it; applications then read it
+ * and recompute no geometry.  This is the reference's residual-to-Jacobian stash (struct EllipStore,
+ * src/fs/tests/ellip.c:119-123, 311, 460-467) with the geometry folded in.  0: recompute the state's
+ * gradient and the geometry at every application (no storage; the memory is returned).  2 (default):
+ * 1 for the nonlinear Jacobian, 0 for the linear operator (p = 2, lambda = 0), the faster choice for
+ * each on MI355X. */
+dErr dohpEllipSetStorage(dohpEllip op, int mode);
 /* Launch shape the autotuner chose on the first application of a mode (0 = linear, 1 = residual,
- * 2 = Jacobian action) and precision: -1 = not tuned yet, 0 = several elements per four-wavefront
+ * 2 = Jacobian action recomputed from the state, 3 = application from stored quadrature data, linear
+ * or Jacobian) and precision: -1 = not tuned yet, 0 = several elements per four-wavefront
  * workgroup, 1 = one-wavefront workgroups, 2 = persistent one-wavefront workgroups with the next
  * batch's loads in flight, 3 = persistent four-wavefront workgroups; ms[4] receives the measured
  * milliseconds of each. */

@@ -132,6 +132,23 @@ def test_ellip_refout_initial_residual_on_gpu(C, name):
     assert rel(F, E.function(E.closure(np.zeros(fs.nowned), dv))[:fs.nowned]) < TOL64
 
 
+@pytest.mark.parametrize("deg", range(1, 11))
+def test_residual_with_forcing_every_degree(C, deg):
+    """The forcing term is assembled on the first residual and reused: two residuals at different states
+    of a manufactured solution (exact solution 0, nonlinear with the Bratu term), each against the oracle."""
+    print(f"forcing p={deg}", flush=True)
+    M = (2, 1, 2) if deg >= 8 else (2, 2, 2)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=17)
+    fs, S, E, op = setup(C, M, deg, V, p=1.6, eps=0.5, lam=0.5, gexact=0, a=1.2, b=0.8, c=1.1)
+    dv = E.dirichlet_values()
+    fs.dirichlet_commit(dv)
+    f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    for seed in (31, 32):
+        x = 0.3 * (2 * splitmix64_uniform(seed, fs.nowned) - 1)
+        op.function(dev(x), f)
+        assert rel(f.cpu().numpy(), E.function(E.closure(x, dv))[:fs.nowned]) < TOL64, seed
+
+
 # ------------------------------------------------- nonlinear residual and Jv
 def sine_state(X, seed):
     """Seeded sum of 8 random-phase sine modes of amplitude 0.1."""
@@ -167,12 +184,74 @@ def test_nonlinear_function_and_jacobian_action(C, M, p, lam):
     assert e32 < 1e-4
 
 
+@pytest.mark.parametrize("deg", range(1, 11))
+def test_stored_quadrature_data_every_degree(C, deg):
+    """J v from stored quadrature data (7 values per point with the Bratu term) at every degree, FP64
+    against the oracle and against the recomputing path, FP32 with its error stated."""
+    print(f"stored p={deg}", flush=True)
+    M = (2, 1, 2) if deg >= 8 else (2, 2, 2)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=23)
+    fs, S, E, op = setup(C, M, deg, V, p=1.6, eps=0.3, lam=0.7)
+    uc = sine_state(S.nodal_coordinates(), 7)
+    fs.dirichlet_commit(uc[fs.nowned:])
+    f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    op.function(dev(uc[:fs.nowned]), f)
+    v = 2 * splitmix64_uniform(13, fs.nowned) - 1
+    ref = E.jacobian_apply(uc, np.concatenate([v, np.zeros(fs.nclosure - fs.nowned)]))[:fs.nowned]
+    y = torch.zeros_like(f)
+    op.mat_mult(dev(v), y)
+    assert rel(y.cpu().numpy(), ref) < TOL64
+    y32 = torch.zeros(fs.nowned, dtype=torch.float32, device="cuda")
+    op.mat_mult_f32(dev(v, torch.float32), y32)
+    e32 = rel(y32.cpu().numpy(), ref)
+    op.set_storage(0)
+    yr = torch.zeros_like(f)
+    op.mat_mult(dev(v), yr)
+    print(f"FP32 error {e32:.2e}; stored vs recomputed {rel(y.cpu().numpy(), yr.cpu().numpy()):.1e}")
+    assert e32 < 1e-4
+    assert rel(y.cpu().numpy(), yr.cpu().numpy()) < TOL64
+
+
+def test_stored_quadrature_data_follows_state_and_coefficient(C):
+    """The stored data is formed again after each residual (new state) and after a new coefficient."""
+    from dohp_amd.mesh import element_vertex_index
+    M, deg = (3, 2, 2), 4
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=29)
+    fs, S, E, op = setup(C, M, deg, V, p=1.7, eps=0.4, lam=0.3)
+    X = S.nodal_coordinates()
+    v = 2 * splitmix64_uniform(37, fs.nowned) - 1
+    vc = np.concatenate([v, np.zeros(fs.nclosure - fs.nowned)])
+    f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(f)
+    y32 = torch.zeros(fs.nowned, dtype=torch.float32, device="cuda")
+    for seed in (3, 4, 5):
+        uc = sine_state(X, seed)
+        fs.dirichlet_commit(uc[fs.nowned:])
+        op.function(dev(uc[:fs.nowned]), f)
+        ref = E.jacobian_apply(uc, vc)[:fs.nowned]
+        if seed != 4:  # the FP32 copy follows too, also when it was not asked for in between
+            op.mat_mult_f32(dev(v, torch.float32), y32)
+            assert rel(y32.cpu().numpy(), ref) < 1e-4, seed
+        op.mat_mult(dev(v), y)
+        assert rel(y.cpu().numpy(), ref) < TOL64, seed
+    lin = C.Ellip(fs, p=2.0)
+    lin.set_storage(1)  # (by default the linear operator recomputes)
+    for seed in (41, 42):
+        kv = np.exp(splitmix64_uniform(seed, V.shape[0]) - 0.5)
+        lin.set_coefficient(kv)
+        El = O.Ellip(S, p=2.0, kvert=kv[element_vertex_index(M)])
+        lin.mat_mult(dev(v), y)
+        assert rel(y.cpu().numpy(), El.jacobian_apply(np.zeros(fs.nclosure), vc)[:fs.nowned]) < TOL64, seed
+
+
+@pytest.mark.parametrize("storage", [0, 1])
 @pytest.mark.parametrize("p", [2, 4, 6, 9])
-def test_every_launch_shape_matches_oracle(C, p):
+def test_every_launch_shape_matches_oracle(C, p, storage):
     """Each launch shape the tuner chooses from (four-wavefront, one-wavefront, and their persistent
     forms), pinned in turn, for the residual, the Jacobian action and the linear apply in FP64 (1e-12)
-    and FP32 (stated error)."""
-    print(f"shapes p={p}", flush=True)
+    and FP32 (stated error), recomputed from the state (storage 0) and from stored quadrature data
+    (storage 1)."""
+    print(f"shapes p={p} storage={storage}", flush=True)
     M = (3, 2, 3)
     V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=17)
     fs, S, E, op = setup(C, M, p, V, p=1.5, eps=np.sqrt(0.005))
@@ -185,12 +264,15 @@ def test_every_launch_shape_matches_oracle(C, p):
     ref_f = E.function(uc)[:fs.nowned]
     ref_j = E.jacobian_apply(uc, vc)[:fs.nowned]
     ref_l = El.jacobian_apply(np.zeros(fs.nclosure), vc)[:fs.nowned]
+    op.set_storage(storage)
+    lin.set_storage(storage)
+    jmode, lmode = ("stored", "stored") if storage else ("jacobian", "linear")
     for shape in range(4):
-        for mode in ("residual", "jacobian"):
+        for mode in ("residual", jmode):
             op.set_shape(mode, 0, shape)
             op.set_shape(mode, 1, shape)
-        lin.set_shape("linear", 0, shape)
-        lin.set_shape("linear", 1, shape)
+        lin.set_shape(lmode, 0, shape)
+        lin.set_shape(lmode, 1, shape)
         f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
         op.function(dev(uc[:fs.nowned]), f)
         y = torch.zeros_like(f)
@@ -206,7 +288,7 @@ def test_every_launch_shape_matches_oracle(C, p):
         print(f"shape {shape}: fp64 {['%.1e' % e for e in errs]} fp32 {['%.1e' % e for e in e32]}")
         assert max(errs) < TOL64, (shape, errs)
         assert max(e32) < 1e-4, (shape, e32)
-        assert op.tuning("jacobian", 0)[0] == shape and lin.tuning("linear", 1)[0] == shape
+        assert op.tuning(jmode, 0)[0] == shape and lin.tuning(lmode, 1)[0] == shape
 
 
 def test_atomic_and_colored_scatter_agree(C):

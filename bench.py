@@ -511,11 +511,14 @@ def main():
     tuning.update({f"linear_stored_{'f32' if f else 'f64'}": lin.tuning("stored", f) for f in (0, 1)})
     lin.set_storage(2)
     # the Jacobian recomputing the state's gradient and the geometry at every application (no storage)
+    ys = torch.zeros_like(y)
+    op.mat_mult(v, ys, stream)
     op.set_storage(0)
     yr = torch.zeros_like(y)
     op.mat_mult(v, yr, stream)
     torch.cuda.synchronize()
-    variants["stored_vs_recomputed_rel_diff"] = float(torch.linalg.norm(yr - y) / torch.linalg.norm(y))
+    variants["stored_vs_recomputed_rel_diff"] = float(torch.linalg.norm(yr - ys) / torch.linalg.norm(ys))
+    del ys, yr
     timed("jv_f64_recomputed", lambda: op.mat_mult(v, y, stream), 8, True, 0)
     timed("jv_f32_recomputed", lambda: op.mat_mult_f32(v32, y32, stream), 4, True, 0)
     tuning.update({f"jacobian_{'f32' if f else 'f64'}": op.tuning("jacobian", f) for f in (0, 1)})

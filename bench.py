@@ -504,7 +504,10 @@ def main():
     tuning = {f"stored_{'f32' if f else 'f64'}": op.tuning("stored", f) for f in (0, 1)}
     tuning["residual_f64"] = op.tuning("residual", 0)
     tuning.update({f"linear_{'f32' if f else 'f64'}": lin.tuning("linear", f) for f in (0, 1)})
-    # the linear operator from stored data (by default it recomputes: faster at this order)
+    # form chosen by the default storage mode (1 stored, 0 recomputed) and its timing [recomputed, stored] ms
+    variants["storage_choice"] = {f"{name}_{'f32' if f else 'f64'}": o.storage(f)
+                                  for name, o in (("jacobian", op), ("linear", lin)) for f in (0, 1)}
+    # the linear operator from stored data (the default keeps the faster form: recomputing, at this order)
     lin.set_storage(1)
     timed("linear_f64_stored", lambda: lin.mat_mult(v, y, stream), 8, False, 6)
     timed("linear_f32_stored", lambda: lin.mat_mult_f32(v32, y32, stream), 4, False, 6)

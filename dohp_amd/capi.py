@@ -71,6 +71,7 @@ _proto("dohpEllipSetShape", _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int)
 _proto("dohpEllipAssembleQ1", _vp, _vp, ctypes.POINTER(_vp), _vp)
 _proto("dohpEllipSetQ1Mode", _vp, ctypes.c_int)
 _proto("dohpEllipSetStorage", _vp, ctypes.c_int)
+_proto("dohpEllipGetStorage", _vp, ctypes.c_int, _vp, _vp)
 _proto("dohpMatGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long))
 _proto("dohpMatGetCSR", _vp, _ip, _ip, _dp)
 _proto("dohpMatMult", _vp, _vp, _vp, _vp)
@@ -398,9 +399,16 @@ class Ellip:
     def set_storage(self, mode):
         """1: applications read per-point quadrature data formed once per linearization (48 bytes
         per point in FP64, 56 with lambda; half again for FP32 applications); 0: recompute from the
-        state and the geometry at every application, no storage; 2 (default): 1 for the nonlinear
-        Jacobian, 0 for the linear operator."""
+        state and the geometry at every application, no storage; 2 (default): the faster of the two,
+        timed on the first application in each precision."""
         check(_lib.dohpEllipSetStorage(self.h, int(mode)))
+
+    def storage(self, fp32=False):
+        """(form in use: 1 stored, 0 recomputed, -1 not timed yet; [ms recomputed, ms stored] of the timing
+        that chose it, zeros unless storage 2 timed it)."""
+        st, ms = ctypes.c_int(), (ctypes.c_float * 2)()
+        check(_lib.dohpEllipGetStorage(self.h, int(fp32), ctypes.byref(st), ms))
+        return st.value, [round(v, 4) for v in ms]
 
     def set_scatter(self, mode):
         check(_lib.dohpEllipSetScatter(self.h, int(mode)))

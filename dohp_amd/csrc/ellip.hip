@@ -70,7 +70,7 @@ constexpr bool jac_state_first()
 
 template <int N, typename T, int MODE, int EPB>
 __global__ void __launch_bounds__(wg_threads(N, EPB))
-__attribute__((amdgpu_waves_per_eu(MODE == MODE_FUN && N <= 7 ? 3 : 1))) ellip_kernel(const int *__restrict__ elist, long ne, BoxMap bm,
+__attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <= 7 ? 3 : 1))) ellip_kernel(const int *__restrict__ elist, long ne, BoxMap bm,
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
                                                     const T *__restrict__ ustate, const T *__restrict__ xdir,
                                                     T *__restrict__ y, long nowned, int atomic,
@@ -354,8 +354,11 @@ struct _p_dohpEllip {
   float shape_ms[kModes][2][4] = {};
   // Jacobian (or linear operator) quadrature data, 6 values per point (7 with the Bratu term), formed
   // on the first application after each residual (the linear operator's once).  storage: 0 = never,
-  // 1 = always, 2 = for the nonlinear Jacobian only (the faster choice for each, measured at n = 7)
+  // 1 = always, 2 = the faster of the two, timed on the first application in each precision (use_qd:
+  // -1 = not timed yet, 0 = recompute, 1 = stored)
   int storage = 2;
+  int use_qd[2] = {-1, -1};
+  float qd_ms[2][2] = {};  // [FP32][recomputed, stored] milliseconds of that timing
   DevBuf<double> qd;
   DevBuf<float> qd32;
   bool qd_ok = false, qd32_ok = false;
@@ -782,28 +785,79 @@ namespace {
 
 // y += J x (A x for the linear operator): from the stored quadrature data, or recomputed from the state
 // and the geometry at every application (storage 0)
-bool stored(const _p_dohpEllip *op) { return op->storage == 1 || (op->storage == 2 && !is_linear(op)); }
+template <typename T>
+struct Prec;
+template <>
+struct Prec<double> {
+  static dErr ensure(_p_dohpEllip *op, hipStream_t st) { return ensure_qd(op, st); }
+  static const double *qd(const _p_dohpEllip *op) { return op->qd.ptr; }
+  static const double *state(const _p_dohpEllip *op) { return op->state.ptr; }
+};
+template <>
+struct Prec<float> {
+  static dErr ensure(_p_dohpEllip *op, hipStream_t st) { return ensure_qd32(op, st); }
+  static const float *qd(const _p_dohpEllip *op) { return op->qd32.ptr; }
+  static const float *state(const _p_dohpEllip *op) { return op->state32.ptr; }
+};
 
-dErr apply_add(_p_dohpEllip *op, const double *x, double *y, hipStream_t st)
+template <typename T>
+dErr apply_recomputed(_p_dohpEllip *op, const T *x, T *y, hipStream_t st)
 {
-  if (!is_linear(op)) DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
-  if (stored(op)) {
-    DOHP_CALL(ensure_qd(op, st));
-    return launch<double, MODE_JQD>(op, op->qd.ptr, x, y, st);
-  }
-  if (is_linear(op)) return launch<double, MODE_LIN>(op, nullptr, x, y, st);
-  return launch<double, MODE_JAC>(op, op->state.ptr, x, y, st);
+  if (is_linear(op)) return launch<T, MODE_LIN>(op, nullptr, x, y, st);
+  return launch<T, MODE_JAC>(op, Prec<T>::state(op), x, y, st);
 }
 
-dErr apply_add(_p_dohpEllip *op, const float *x, float *y, hipStream_t st)
+template <typename T>
+dErr apply_stored(_p_dohpEllip *op, const T *x, T *y, hipStream_t st)
+{
+  DOHP_CALL(Prec<T>::ensure(op, st));
+  return launch<T, MODE_JQD>(op, Prec<T>::qd(op), x, y, st);
+}
+
+// milliseconds of one call of f on stream st (after one untimed call, which also tunes its launch shape)
+template <typename F>
+dErr time_call(F f, hipStream_t st, float &ms)
+{
+  hipEvent_t e0, e1;
+  DOHP_CALL(f());
+  DOHP_HIP(hipEventCreate(&e0));
+  DOHP_HIP(hipEventCreate(&e1));
+  DOHP_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < 2; r++) DOHP_CALL(f());
+  DOHP_HIP(hipEventRecord(e1, st));
+  DOHP_HIP(hipEventSynchronize(e1));
+  DOHP_HIP(hipEventElapsedTime(&ms, e0, e1));
+  ms /= 2;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+// y += J x (A x for the linear operator): from the stored quadrature data or recomputed from the state and
+// the geometry.  With storage 2 the first application in each precision times both forms on a scratch
+// output and keeps the faster (per application; forming the data is paid once per linearization).
+template <typename T>
+dErr apply_add(_p_dohpEllip *op, const T *x, T *y, hipStream_t st)
 {
   if (!is_linear(op)) DOHP_CHECK(op->has_state, dohpERR_ARG, "no linearization state: evaluate the function first");
-  if (stored(op)) {
-    DOHP_CALL(ensure_qd32(op, st));
-    return launch<float, MODE_JQD>(op, op->qd32.ptr, x, y, st);
+  const int p32 = sizeof(T) == 4;
+  int use = op->storage == 2 ? op->use_qd[p32] : op->storage;
+  if (use < 0) {
+    DevBuf<T> scratch;
+    DOHP_CALL(scratch.alloc(op->fs->nowned > 0 ? op->fs->nowned : 1));
+    float t_qd = 0, t_re = 0;
+    DOHP_CALL(time_call([&]() { return apply_stored<T>(op, x, scratch.ptr, st); }, st, t_qd));
+    DOHP_CALL(time_call([&]() { return apply_recomputed<T>(op, x, scratch.ptr, st); }, st, t_re));
+    use = op->use_qd[p32] = t_qd < t_re ? 1 : 0;
+    op->qd_ms[p32][0] = t_re;
+    op->qd_ms[p32][1] = t_qd;
+    if (!use && op->use_qd[1 - p32] != 1) {  // neither precision reads it: the memory is returned
+      drop_qd(op);
+      op->qd.release();
+      op->qd32.release();
+    }
   }
-  if (is_linear(op)) return launch<float, MODE_LIN>(op, nullptr, x, y, st);
-  return launch<float, MODE_JAC>(op, op->state32.ptr, x, y, st);
+  return use ? apply_stored<T>(op, x, y, st) : apply_recomputed<T>(op, x, y, st);
 }
 
 }  // namespace
@@ -831,12 +885,24 @@ extern "C" dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x, float *y,
   return apply_add(op, x, y, st);
 }
 
+extern "C" dErr dohpEllipGetStorage(dohpEllip op, int fp32, int *stored, float *ms)
+{
+  DOHP_CHECK(op && (fp32 == 0 || fp32 == 1), dohpERR_ARG, "bad argument");
+  if (stored) *stored = op->storage == 2 ? op->use_qd[fp32] : op->storage;
+  if (ms) {
+    ms[0] = op->qd_ms[fp32][0];
+    ms[1] = op->qd_ms[fp32][1];
+  }
+  return 0;
+}
+
 extern "C" dErr dohpEllipSetStorage(dohpEllip op, int mode)
 {
   DOHP_CHECK(op && mode >= 0 && mode <= 2, dohpERR_ARG,
-             "storage must be 0 (recompute), 1 (quadrature data) or 2 (quadrature data for the nonlinear Jacobian)");
+             "storage must be 0 (recompute), 1 (quadrature data) or 2 (the faster, timed)");
   op->storage = mode;
-  if (!stored(op)) {  // the memory is returned
+  op->use_qd[0] = op->use_qd[1] = -1;
+  if (mode == 0) {  // the memory is returned
     drop_qd(op);
     op->qd.release();
     op->qd32.release();

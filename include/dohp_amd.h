@@ -121,9 +121,15 @@ dErr dohpEllipSetScatter(dohpEllip op, int mode);
  * and recompute no geometry.  This is the reference's residual-to-Jacobian stash (struct EllipStore,
  * src/fs/tests/ellip.c:119-123, 311, 460-467) with the geometry folded in.  0: recompute the state's
  * gradient and the geometry at every application (no storage; the memory is returned).  2 (default):
- * 1 for the nonlinear Jacobian, 0 for the linear operator (p = 2, lambda = 0), the faster choice for
- * each on MI355X. */
+ * the first application in each precision times both forms (each with its tuned launch shape) and
+ * keeps the faster per application; the memory is returned when neither precision keeps the stored
+ * form.  At 64^3 hexahedra, p = 6 the nonlinear Jacobian keeps the stored form and the linear operator
+ * (p = 2, lambda = 0) recomputes. */
 dErr dohpEllipSetStorage(dohpEllip op, int mode);
+/* The form that applications in a precision use (1 = stored quadrature data, 0 = recomputed, -1 = with
+ * storage 2, not timed yet) and, with storage 2, the timing behind the choice: ms[0] recomputed, ms[1]
+ * from stored data (milliseconds per application; 0 until timed). */
+dErr dohpEllipGetStorage(dohpEllip op, int fp32, int *stored, float *ms);
 /* Launch shape the autotuner chose on the first application of a mode (0 = linear, 1 = residual,
  * 2 = Jacobian action recomputed from the state, 3 = application from stored quadrature data, linear
  * or Jacobian) and precision: -1 = not tuned yet, 0 = several elements per four-wavefront

@@ -235,13 +235,46 @@ def test_stored_quadrature_data_follows_state_and_coefficient(C):
         op.mat_mult(dev(v), y)
         assert rel(y.cpu().numpy(), ref) < TOL64, seed
     lin = C.Ellip(fs, p=2.0)
-    lin.set_storage(1)  # (by default the linear operator recomputes)
+    lin.set_storage(1)  # (by default the faster form is timed and kept: here the stored one is wanted)
     for seed in (41, 42):
         kv = np.exp(splitmix64_uniform(seed, V.shape[0]) - 0.5)
         lin.set_coefficient(kv)
         El = O.Ellip(S, p=2.0, kvert=kv[element_vertex_index(M)])
         lin.mat_mult(dev(v), y)
         assert rel(y.cpu().numpy(), El.jacobian_apply(np.zeros(fs.nclosure), vc)[:fs.nowned]) < TOL64, seed
+
+
+def test_storage_choice_is_timed_and_kept(C):
+    """Storage 2 (default) times both forms on the first application in each precision, keeps the faster,
+    and the result matches the oracle whichever it kept; 0 and 1 report themselves."""
+    print("storage choice", flush=True)
+    M, deg = (4, 3, 3), 5
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=31)
+    fs, S, E, op = setup(C, M, deg, V, p=1.5, eps=0.2)
+    uc = sine_state(S.nodal_coordinates(), 8)
+    fs.dirichlet_commit(uc[fs.nowned:])
+    f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    op.function(dev(uc[:fs.nowned]), f)
+    v = 2 * splitmix64_uniform(43, fs.nowned) - 1
+    ref = E.jacobian_apply(uc, np.concatenate([v, np.zeros(fs.nclosure - fs.nowned)]))[:fs.nowned]
+    assert op.storage(False)[0] == -1 and op.storage(True)[0] == -1
+    y = torch.full((fs.nowned,), 3.0, dtype=torch.float64, device="cuda")  # overwritten
+    op.mat_mult(dev(v), y)
+    y32 = torch.zeros(fs.nowned, dtype=torch.float32, device="cuda")
+    op.mat_mult_f32(dev(v, torch.float32), y32)
+    for fp32 in (False, True):
+        use, ms = op.storage(fp32)
+        print("fp32" if fp32 else "fp64", use, ms)
+        assert use in (0, 1) and min(ms) > 0 and use == int(ms[1] < ms[0])
+    assert rel(y.cpu().numpy(), ref) < TOL64
+    assert rel(y32.cpu().numpy(), ref) < 1e-4
+    op.mat_mult(dev(v), y)  # the choice is kept: same result
+    assert rel(y.cpu().numpy(), ref) < TOL64
+    for mode in (0, 1):
+        op.set_storage(mode)
+        assert op.storage(False)[0] == mode
+        op.mat_mult(dev(v), y)
+        assert rel(y.cpu().numpy(), ref) < TOL64, mode
 
 
 @pytest.mark.parametrize("storage", [0, 1])

@@ -452,8 +452,10 @@ def main():
     y32 = torch.zeros_like(v32)
     stream = torch.cuda.current_stream()
     op.function(u, f, stream)
-    op.mat_mult(v, y, stream)  # the linearization's quadrature data is formed here (reported below)
+    op.mat_mult(v, y, stream)  # the storage choice is timed and the quadrature data formed here (reported below)
+    op.mat_mult_f32(v32, y32, stream)
     torch.cuda.synchronize()
+    stored64, stored32 = op.storage(False)[0], op.storage(True)[0]
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f}s: {fs.nelem} elements p={p}, N_int={fs.nowned}, N_tot={fs.nclosure}",
           file=sys.stderr, flush=True)
@@ -493,7 +495,7 @@ def main():
     variants["quadrature_data_f64"] = {"us": round((t_pair - t_res - t_step) * 1e6, 1),
                                        "bytes": 6 * 8 * n ** 3 * fs.nelem}
     print(f"# quadrature data {(t_pair - t_res - t_step) * 1e6:.1f} us", file=sys.stderr, flush=True)
-    timed("jv_f32", lambda: op.mat_mult_f32(v32, y32, stream), 4, True, 6)
+    timed("jv_f32", lambda: op.mat_mult_f32(v32, y32, stream), 4, True, 6 if stored32 == 1 else 0)
     timed("linear_f64", lambda: lin.mat_mult(v, y, stream), 8, False, 0)
     timed("linear_f32", lambda: lin.mat_mult_f32(v32, y32, stream), 4, False, 0)
     # FP32 error against the FP64 apply (same inputs)
@@ -528,9 +530,10 @@ def main():
     variants["tuning"] = tuning
     op.set_storage(2)
 
-    B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True, nq=6)
+    B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True, nq=6 if stored64 == 1 else 0)
     achieved = B / (t_kernel or t_step) / 1e9
-    workload = f"nonlinear diffusion J(u0)v from stored quadrature data, {Me}^3 hex, p={p}, FP64, perturbed mesh"
+    form = "from stored quadrature data, " if stored64 == 1 else ""
+    workload = f"nonlinear diffusion J(u0)v {form}{Me}^3 hex, p={p}, FP64, perturbed mesh"
     cpu = None
     if not args.no_cpu:
         ncores = os.cpu_count() or 1
@@ -555,7 +558,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "elements": fs.nelem, "degree": p, "n_int": fs.nowned,
                    "n_tot": fs.nclosure, "scatter": "colored" if args.scatter == 0 else "atomic",
-                   "jacobian_data": "stored per Gauss point (6 FP64 values)"},
+                   "jacobian_data": "stored per Gauss point (6 FP64 values)" if stored64 == 1 else "recomputed"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload),
                      "kernel_us": round((t_kernel or t_step) * 1e6, 1), "algorithmic_bytes": int(B)},

@@ -846,9 +846,18 @@ dErr apply_add(_p_dohpEllip *op, const T *x, T *y, hipStream_t st)
     DevBuf<T> scratch;
     DOHP_CALL(scratch.alloc(op->fs->nowned > 0 ? op->fs->nowned : 1));
     float t_qd = 0, t_re = 0;
-    DOHP_CALL(time_call([&]() { return apply_stored<T>(op, x, scratch.ptr, st); }, st, t_qd));
+    // the data may not fit in device memory: then the recomputing form is kept
+    const bool fits = Prec<T>::ensure(op, st) == 0;
+    if (fits) {
+      DOHP_CALL(time_call([&]() { return apply_stored<T>(op, x, scratch.ptr, st); }, st, t_qd));
+    } else {
+      (void)hipGetLastError();  // the failed allocation is not an error of the application
+      drop_qd(op);
+      op->qd.release();
+      op->qd32.release();
+    }
     DOHP_CALL(time_call([&]() { return apply_recomputed<T>(op, x, scratch.ptr, st); }, st, t_re));
-    use = op->use_qd[p32] = t_qd < t_re ? 1 : 0;
+    use = op->use_qd[p32] = fits && t_qd < t_re ? 1 : 0;
     op->qd_ms[p32][0] = t_re;
     op->qd_ms[p32][1] = t_qd;
     if (!use && op->use_qd[1 - p32] != 1) {  // neither precision reads it: the memory is returned

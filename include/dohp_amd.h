@@ -122,8 +122,8 @@ dErr dohpEllipSetScatter(dohpEllip op, int mode);
  * src/fs/tests/ellip.c:119-123, 311, 460-467) with the geometry folded in.  0: recompute the state's
  * gradient and the geometry at every application (no storage; the memory is returned).  2 (default):
  * the first application in each precision times both forms (each with its tuned launch shape) and
- * keeps the faster per application; the memory is returned when neither precision keeps the stored
- * form.  At 64^3 hexahedra, p = 6 the nonlinear Jacobian keeps the stored form and the linear operator
+ * keeps the faster per application (the recomputed one when the data does not fit in device memory);
+ * the memory is returned when neither precision keeps the stored form.  At 64^3 hexahedra, p = 6 the nonlinear Jacobian keeps the stored form and the linear operator
  * (p = 2, lambda = 0) recomputes. */
 dErr dohpEllipSetStorage(dohpEllip op, int mode);
 /* The form that applications in a precision use (1 = stored quadrature data, 0 = recomputed, -1 = with

@@ -177,15 +177,17 @@ def solve_workload(args):
     q1c = op.assemble_q1(torch.zeros_like(xs))
     op.set_q1_mode(0)
     res = {}
-    for name, mixed, qq in (("plain_f64", 0, q1), ("mixed_f32_f64", 1, q1), ("f64_cg_f32_precond", 2, q1),
-                            ("corrected_q1_plain_f64", 0, q1c), ("corrected_q1_mixed_f32_f64", 1, q1c),
-                            ("corrected_q1_f64_cg_f32_precond", 2, q1c)):
+    # (runs named *_mg1 smooth with degree 1 instead of the default 2: faster here, not robust everywhere)
+    for name, mixed, qq, mgd in (("plain_f64", 0, q1, 2), ("mixed_f32_f64", 1, q1, 2), ("f64_cg_f32_precond", 2, q1, 2),
+                                 ("corrected_q1_plain_f64", 0, q1c, 2), ("corrected_q1_mixed_f32_f64", 1, q1c, 2),
+                                 ("corrected_q1_f64_cg_f32_precond", 2, q1c, 2), ("f64_cg_f32_precond_mg1", 2, q1, 1),
+                                 ("corrected_q1_f64_cg_f32_precond_mg1", 2, q1c, 1)):
         x = torch.zeros_like(xs)
-        op.solve(qq, b, x, mixed=mixed, rtol=1e-10)  # warm-up (kernels, allocations)
+        op.solve(qq, b, x, mixed=mixed, rtol=1e-10, mg_degree=mgd)  # warm-up (kernels, allocations)
         x.zero_()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        st = op.solve(qq, b, x, mixed=mixed, rtol=1e-10)
+        st = op.solve(qq, b, x, mixed=mixed, rtol=1e-10, mg_degree=mgd)
         torch.cuda.synchronize()
         t = time.perf_counter() - t
         r = torch.zeros_like(x)

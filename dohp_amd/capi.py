@@ -105,7 +105,7 @@ class MG:
         check(_lib.dohpMGApplyF32(self.h, _ptr(r), _ptr(z), _stream(stream)))
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if _lib is not None and getattr(self, "h", None):
             _lib.dohpMGDestroy(ctypes.byref(self.h))
 
 
@@ -212,7 +212,7 @@ class MLP:
         return E.value
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if _lib is not None and getattr(self, "h", None):
             _lib.dohpMLPDestroy(ctypes.byref(self.h))
 
 
@@ -316,7 +316,7 @@ class FS:
         check(_lib.dFSExpandedToGlobal(self.h, _ptr(x), _ptr(g), hmode, imode, _stream(stream)))
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if _lib is not None and getattr(self, "h", None):
             _lib.dFSDestroy(ctypes.byref(self.h))
             self.h = None
 
@@ -356,7 +356,7 @@ class Mat:
         check(_lib.dohpMatGetDiagonal(self.h, _ptr(d), _stream(stream)))
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if _lib is not None and getattr(self, "h", None):
             _lib.dohpMatDestroy(ctypes.byref(self.h))
             self.h = None
 
@@ -466,7 +466,7 @@ class Ellip:
         return out
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if _lib is not None and getattr(self, "h", None):
             _lib.dohpEllipDestroy(ctypes.byref(self.h))
             self.h = None
 
@@ -554,6 +554,6 @@ class Batch:
         return t.value, [round(v, 4) for v in ms]
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if _lib is not None and getattr(self, "h", None):
             _lib.dohpBatchDestroy(ctypes.byref(self.h))
             self.h = None

@@ -123,7 +123,8 @@ dErr dohpEllipSetScatter(dohpEllip op, int mode);
  * gradient and the geometry at every application (no storage; the memory is returned).  2 (default):
  * the first application in each precision times both forms (each with its tuned launch shape) and
  * keeps the faster per application (the recomputed one when the data does not fit in device memory);
- * the memory is returned when neither precision keeps the stored form.  At 64^3 hexahedra, p = 6 the nonlinear Jacobian keeps the stored form and the linear operator
+ * the memory is returned when neither precision keeps the stored form.  That first application waits
+ * for its stream (it times launches), as the launch-shape tuner's first application of a mode does.  At 64^3 hexahedra, p = 6 the nonlinear Jacobian keeps the stored form and the linear operator
  * (p = 2, lambda = 0) recomputes. */
 dErr dohpEllipSetStorage(dohpEllip op, int mode);
 /* The form that applications in a precision use (1 = stored quadrature data, 0 = recomputed, -1 = with

@@ -330,11 +330,16 @@ __device__ inline double rsqrt_nr(double x)
 }
 __device__ inline float rsqrt_nr(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// The general power, outside the kernels' inlined code: the quarter-integer exponents of the common p take the
+// fast path of powe, and this call's registers do not constrain the allocation of the rest.
+__device__ __noinline__ double pow_general(double x, double e) { return pow(x, e); }
+__device__ __noinline__ float pow_general(float x, float e) { return powf(x, e); }
+
 template <typename T>
 __device__ inline T powe(T x, T e, int qpow)
 {
   if (qpow == 0) return T(1);
-  if (qpow > 8 || qpow < -8) return pow(x, e);
+  if (qpow > 8 || qpow < -8) return pow_general(x, e);
   const T r2 = rsqrt_nr(x);         // x^(-1/2)
   const T r4 = r2 * rsqrt_nr(r2);   // x^(-1/4)
   const int n = qpow < 0 ? -qpow : qpow;

@@ -438,12 +438,12 @@ class Ellip:
         return m
 
     def solve(self, q1, b, x, mixed=False, rtol=1e-10, max_it=2000, inner_rtol=1e-4, max_outer=50, cheb_degree=4,
-              cheb_ratio=30.0, eig_its=20, pc="mg", mg_degree=2, mlp=None, fallback_its=0, record=None,
+              cheb_ratio=30.0, eig_its=20, pc="mg", mg_degree=0, mlp=None, fallback_its=0, record=None,
               stream=None):
         """CG on the linear operator, preconditioned through the embedded-Q1 matrix: pc="mg" is a
         multigrid V-cycle with Jacobi-Chebyshev smoothing, pc="cheb" Jacobi-Chebyshev alone.
         mixed: 0 plain FP64, 1 FP32 inner PCG inside FP64 defect correction, 2 FP64 PCG with the
-        preconditioner in FP32.  mlp: learned node weights around the V-cycle (plain fallback after
+        preconditioner in FP32.  mg_degree: smoothing degree, 0 = 1 on a corrected Q1 matrix, 2 otherwise.  mlp: learned node weights around the V-cycle (plain fallback after
         fallback_its iterations).  record=(snap, every): plain FP64 solve keeping iterates."""
         o = SolveOpts(int(mixed), float(rtol), int(max_it), float(inner_rtol), int(max_outer), int(cheb_degree),
                       float(cheb_ratio), int(eig_its), {"cheb": 0, "mg": 1}[pc], int(mg_degree), int(fallback_its))

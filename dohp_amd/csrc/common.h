@@ -118,6 +118,7 @@ struct _p_dohpMat {
   // matrix is refilled
   dohpMG mg = nullptr;
   int mg_degree = 0;
+  int corrected = 0;  // assembled with the corrected sub-cell arithmetic (embedded-Q1 mode 1)
   ~_p_dohpMat() { dohpMGDestroy(&mg); }
 };
 dErr dohpMatBuildStencil_Private(_p_dohpMat *A, dohpFS fs, hipStream_t st);

@@ -352,6 +352,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   tb.p = fs->p;
   tb.n = fs->n;
   tb.corrected = q1mode;
+  A->corrected = q1mode;
   {
     double zg[kMaxN], wg[kMaxN];
     gauss_lobatto(fs->n, zg, wg);

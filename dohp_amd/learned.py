@@ -62,7 +62,9 @@ def collect_pairs(C, torch, seeds, M=(12, 12, 12), deg=6, every=4, nsnap=8, nran
             r = torch.zeros_like(e)
             op.mat_mult(e, r)  # r = J e (= b - J x_k for the iterates)
             pairs.append((r, e))
-        probs.append({"fs": fs, "op": op, "q1": q1, "mg": C.MG(q1), "X": problem_features(C, torch, fs, q1),
+        # the V-cycle of the solves (smoothing degree 1 on the corrected Q1 matrix, 2 on the reference's)
+        mg = C.MG(q1, smooth_degree=1 if q1_mode else 2)
+        probs.append({"fs": fs, "op": op, "q1": q1, "mg": mg, "X": problem_features(C, torch, fs, q1),
                       "pairs": pairs, "b": b, "its": st["inner_its"]})
     return probs
 

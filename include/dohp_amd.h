@@ -283,9 +283,10 @@ typedef struct {
   double cheb_ratio;  /* Chebyshev interval [lmax/ratio, lmax] of D^{-1} A_q1 */
   int eig_its;        /* power iterations for lmax */
   int pc;             /* 0: Jacobi-Chebyshev on the Q1 matrix; 1: multigrid V-cycle on it (dohpMG) */
-  int mg_degree;      /* Chebyshev degree of the multigrid smoother (pre and post; 0 = 2).  1 is faster where
-                         it converges (48^3, p = 5, corrected Q1: 0.08 s against 0.14 s) but can stall with an
-                         FP32 preconditioner (48^3, p = 6, reference Q1: 489 iterations against 55) */
+  int mg_degree;      /* Chebyshev degree of the multigrid smoother (pre and post).  0: 1 on a corrected
+                         embedded-Q1 matrix (dohpEllipSetQ1Mode 1; 48^3, p = 5: 0.08 s against 0.14 s at
+                         degree 2), 2 on the reference arithmetic's, where degree 1 with an FP32
+                         preconditioner can stall (48^3, p = 6: 489 iterations against 55) */
   int fallback_its;   /* learned solve: iterations after which the plain preconditioner takes over (0 = never) */
 } dohpSolveOpts;
 

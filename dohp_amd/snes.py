@@ -5,6 +5,8 @@ embedded-Q1 preconditioning matrix (EllipJacobian) with PETSc's SNES and KSP (sr
 SNES rtol 1e-10, KSP and line search from the options database).  PETSc is not part of this build, so this
 module restates the two roles that driver relies on, with PETSc's default semantics:
 
+- `cg`: preconditioned conjugate gradients (KSPCG defaults: the preconditioned residual norm is monitored and
+  tested);
 - `gmres`: GMRES(restart) with left preconditioning from a zero initial guess (KSPGMRES defaults: classical
   Gram-Schmidt, restart 30; converged when the preconditioned residual norm falls to
   max(rtol * ||M^-1 b||, atol), diverged past dtol * ||M^-1 b||), returning the monitored norm history
@@ -103,6 +105,46 @@ def gmres(A, M, b, rtol=1e-5, atol=1e-50, dtol=1e5, restart=30, max_it=10000):
             return x, hist, 2 if beta <= atol else 1
 
 
+def cg(A, M, b, rtol=1e-5, atol=1e-50, dtol=1e5, max_it=10000):
+    """Preconditioned conjugate gradients from x0 = 0 with KSPCG's defaults: the monitored and tested norm is
+    the preconditioned residual ||M^-1 r||, converged at max(rtol * ||M^-1 b||, atol).  A and M as in gmres
+    (both symmetric positive definite).  Returns (x, norm history, reason)."""
+    x = torch.zeros_like(b)
+    r = b.clone()
+    z = torch.empty_like(b)
+    q = torch.empty_like(b)
+    if M is None:
+        z.copy_(r)
+    else:
+        M(r, z)
+    dp = _norm(z)
+    hist = [dp]
+    ttol = max(rtol * dp, atol)
+    if dp <= ttol:
+        return x, hist, 2 if dp <= atol else 1
+    p = z.clone()
+    rz = float(torch.dot(r, z))
+    for it in range(1, max_it + 1):
+        A(p, q)
+        alpha = rz / float(torch.dot(p, q))
+        x += alpha * p
+        r -= alpha * q
+        if M is None:
+            z.copy_(r)
+        else:
+            M(r, z)
+        dp = _norm(z)
+        hist.append(dp)
+        if dp <= ttol:
+            return x, hist, 2 if dp <= atol else 1
+        if dp > dtol * hist[0]:
+            return x, hist, -2
+        rz_new = float(torch.dot(r, z))
+        p.mul_(rz_new / rz).add_(z)
+        rz = rz_new
+    return x, hist, -1
+
+
 def _line_search_bt(op, x, f, fnorm, Y, t, alpha=1e-4, minlambda=1e-12):
     """PETSc's backtracking line search (order 3): the full step if ||F(x - Y)||^2 <= ||F||^2 + 2 alpha
     slope, else a quadratic then cubic model of ||F(x - lambda Y)||^2 / 2 along the step, with lambda kept
@@ -140,12 +182,13 @@ def _line_search_bt(op, x, f, fnorm, Y, t, alpha=1e-4, minlambda=1e-12):
 
 
 def newton(op, x, pc=None, rtol=1e-10, atol=1e-50, stol=1e-8, max_it=50, linesearch="basic", ksp_rtol=1e-5,
-           ksp_atol=1e-50, ksp_restart=30, ksp_max_it=10000):
+           ksp_atol=1e-50, ksp_restart=30, ksp_max_it=10000, ksp_type="gmres"):
     """Solve F(x) = 0 for an operator with op.function(x, f) (the residual, which also records x as the
     linearization state) and op.mat_mult(v, y) (the Jacobian action at that state).  x: the initial guess
     on the owned nodes (updated in place).  pc: None (unpreconditioned), "mg" (multigrid V-cycle on the
     embedded-Q1 matrix assembled at each iterate; op.assemble_q1) or a callable pc(op, x) returning
-    M(v, out) for the current iterate.  Returns (x, ||F|| history, KSP histories, reason)."""
+    M(v, out) for the current iterate; ksp_type "gmres" or "cg".  Returns (x, ||F|| history, KSP histories,
+    reason)."""
     f = torch.empty_like(x)
     t = torch.empty_like(x)
     op.function(x, f)
@@ -165,7 +208,11 @@ def newton(op, x, pc=None, rtol=1e-10, atol=1e-50, stol=1e-8, max_it=50, linesea
             M = mg.apply
         else:
             M = pc(op, x)
-        Y, kh, kreason = gmres(op.mat_mult, M, f, rtol=ksp_rtol, atol=ksp_atol, restart=ksp_restart, max_it=ksp_max_it)
+        if ksp_type == "cg":
+            Y, kh, kreason = cg(op.mat_mult, M, f, rtol=ksp_rtol, atol=ksp_atol, max_it=ksp_max_it)
+        else:
+            Y, kh, kreason = gmres(op.mat_mult, M, f, rtol=ksp_rtol, atol=ksp_atol, restart=ksp_restart,
+                                   max_it=ksp_max_it)
         ksp_hist.append(kh)
         if linesearch == "basic":
             x -= Y

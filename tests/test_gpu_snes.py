@@ -103,3 +103,23 @@ def test_newton_with_multigrid_preconditioner(C):
     err = np.abs(x.cpu().numpy() - exact).max() / np.abs(exact).max()
     print("max nodal error", err)  # 3.7e-9 for the same Newton iteration on the oracle with LU
     assert err < 1e-6
+
+
+def test_cg_reproduces_the_morph_history(C):
+    """ellip-morph (src/fs/tests/CMakeLists.txt:57: -morph 1 -twist 0.4 -snes_max_it 1 -ksp_type cg -ksp_rtol 1e-8
+    -pc_type lu): the 31 preconditioned residual norms of the CG solve of the one Newton step, with the GPU
+    operator on the curved, twisted mesh."""
+    from dohp_amd import snes
+    print("ellip-morph cg", flush=True)
+    M = (4, 4, 4)
+    V = block_vertices(M, morph=1.0, twist=0.4)
+    fs = C.FS(M, 3, V)
+    prm = dict(p=2.0, eps=1.0, lam=0.0, exact=0, a=1.0, b=1.0, c=1.0)
+    op = C.Ellip(fs, **prm)
+    fs.dirichlet_commit(O.Ellip(O.BlockSpace(M, 3, V), **prm).dirichlet_values())
+    x = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    x, hist, ksp, reason = snes.newton(op, x, pc=lu_pc, max_it=1, ksp_type="cg", ksp_rtol=1e-8)
+    ref = HIST["ellip-morph"]["ksp"][0]
+    print("cg", ["%.6g" % h for h in ksp[0]])
+    assert len(ksp[0]) == len(ref)
+    match(ksp[0], ref, ref[0])

@@ -67,3 +67,18 @@ def test_oracle_newton_reproduces_the_reference_histories(name):
     assert [len(k) for k in ksp] == [len(k) for k in ref["ksp"]]
     for kh, kr in zip(ksp, ref["ksp"]):
         match(kh, kr, kr[0])
+
+
+def test_oracle_cg_reproduces_the_morph_history():
+    """ellip-morph (src/fs/tests/CMakeLists.txt:57: -morph 1 -twist 0.4 -snes_max_it 1 -ksp_type cg -ksp_rtol 1e-8
+    -pc_type lu): the 31 preconditioned residual norms of the one Newton step's CG solve on the curved mesh."""
+    M = (4, 4, 4)
+    V = block_vertices(M, morph=1.0, twist=0.4)
+    S = O.BlockSpace(M, 3, V)
+    E = O.Ellip(S, p=2.0, eps=1.0, lam=0.0, exact=0, a=1.0, b=1.0, c=1.0)
+    op = OracleOp(E, S, E.dirichlet_values())
+    x, hist, ksp, reason = snes.newton(op, torch.zeros(S.nowned, dtype=torch.float64), pc=lu_pc, max_it=1,
+                                       ksp_type="cg", ksp_rtol=1e-8)
+    ref = HIST["ellip-morph"]["ksp"][0]
+    assert len(ksp[0]) == len(ref)
+    match(ksp[0], ref, ref[0])

@@ -1,6 +1,7 @@
 """The Newton-Krylov layer (dohp_amd/snes.py) on small CPU systems, pinned by what the mathematics fixes:
-GMRES's k-th residual norm is the least-squares minimum over the k-th Krylov space (computed here
-independently from an explicit Krylov basis), restarted GMRES reaches the solution, Newton converges
+GMRES's k-th residual norm is the least-squares minimum over the k-th Krylov space and PCG's k-th iterate the
+energy minimum over it (both computed here independently from an explicit Krylov basis), restarted GMRES
+reaches the solution, Newton converges
 quadratically near a root, and the backtracking line search takes a step that satisfies its sufficient-
 decrease condition where the full Newton step would increase ||F||."""
 import numpy as np
@@ -107,3 +108,27 @@ def test_backtracking_line_search_decreases_where_the_full_step_fails():
     assert reason > 0 and hist[-1] < 1e-10 * hist[0]
     x, hist, _, reason = snes.newton(Atan(), x0.clone(), linesearch="basic", ksp_rtol=1e-12, max_it=8)
     assert hist[-1] > hist[0]  # the full step diverges from this start
+
+
+def test_cg_matches_the_krylov_energy_minimum():
+    """PCG's iterate k minimizes the A-norm of the error over x0 + K_k(M^-1 A, M^-1 b); the iterates agree with
+    that minimum computed from an explicit Krylov basis, and the monitored norm is ||M^-1 r||."""
+    rng = np.random.default_rng(11)
+    n = 30
+    R = rng.normal(size=(n, n))
+    A = R @ R.T + n * np.eye(n)
+    Minv = np.diag(1 / np.diag(A))
+    b = rng.normal(size=n)
+    xstar = np.linalg.solve(A, b)
+    for k in (1, 3, 6):
+        x, hist, reason = snes.cg(matvec(A), matvec(Minv), torch.as_tensor(b), rtol=0.0, max_it=k)
+        K = [Minv @ b]
+        for _ in range(k - 1):
+            K.append(Minv @ A @ K[-1])
+        Q, _ = np.linalg.qr(np.array(K).T)
+        y = np.linalg.solve(Q.T @ A @ Q, Q.T @ b)  # Galerkin = energy minimum
+        np.testing.assert_allclose(x.numpy(), Q @ y, rtol=1e-8, atol=1e-10)
+        assert hist[-1] == pytest.approx(np.linalg.norm(Minv @ (b - A @ x.numpy())), rel=1e-8)
+    x, hist, reason = snes.cg(matvec(A), matvec(Minv), torch.as_tensor(b), rtol=1e-12)
+    assert reason == 1
+    np.testing.assert_allclose(x.numpy(), xstar, rtol=1e-9)

@@ -11,7 +11,8 @@ import bench  # noqa: E402
 
 
 def test_workloads_exist():
-    for name in ("solve_workload", "learned_workload", "batch_workload", "batch_problem", "cpu_baseline", "copy_probe"):
+    for name in ("solve_workload", "learned_workload", "batch_workload", "batch_problem", "newton_workload", "cpu_baseline",
+                 "copy_probe"):
         assert callable(getattr(bench, name)), name
 
 

@@ -121,7 +121,10 @@ struct _p_dohpMat {
   int corrected = 0;  // assembled with the corrected sub-cell arithmetic (embedded-Q1 mode 1)
   ~_p_dohpMat() { dohpMGDestroy(&mg); }
 };
-dErr dohpMatBuildStencil_Private(_p_dohpMat *A, dohpFS fs, hipStream_t st);
+// Sets the box of a block-space matrix and allocates its stencil form (*box = false: not a box, CSR only).
+dErr dohpMatPrepareStencil_Private(_p_dohpMat *A, dohpFS fs, bool *box, hipStream_t st);
+// CSR values (FP64, FP32) and the FP32 stencil from the FP64 stencil form.
+dErr dohpMatStencilToCSR_Private(_p_dohpMat *A, hipStream_t st);
 dErr dohpStencilMult_Private(const _p_dohpMat *A, const double *x, double *y, hipStream_t st);
 dErr dohpStencilMultF32_Private(const _p_dohpMat *A, const float *x, float *y, hipStream_t st);
 

@@ -52,28 +52,25 @@ __device__ __forceinline__ void unrank(const Box &b, long r, int &I, int &J, int
   K = (int)(rem - (long)J * b.nz);
 }
 
-// CSR -> stencil (rows are box-ordered owned nodes, columns their owned neighbours).
-__global__ void csr_to_stencil_kernel(Box b, const int *__restrict__ rowptr, const int *__restrict__ col,
-                                      const double *__restrict__ val, double *__restrict__ st,
-                                      float *__restrict__ st32)
+// stencil -> CSR values (and the FP32 copies of both).
+__global__ void stencil_to_csr_kernel(Box b, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                      const double *__restrict__ st, double *__restrict__ val,
+                                      float *__restrict__ val32, float *__restrict__ st32)
 {
   const long n = b.n();
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
-  for (int s = 0; s < 27; s++) {
-    st[s * n + r] = 0.0;
-    st32[s * n + r] = 0.0f;
-  }
+  for (int s = 0; s < 27; s++) st32[s * n + r] = (float)st[s * n + r];
   int I, J, K;
   unrank(b, r, I, J, K);
   for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
     int I2, J2, K2;
     unrank(b, col[k], I2, J2, K2);
     const int di = I2 - I, dj = J2 - J, dk = K2 - K;
-    if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;  // not a lattice neighbour
-    const int s = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
-    st[s * n + r] = val[k];
-    st32[s * n + r] = (float)val[k];
+    double v = 0.0;
+    if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1) v = st[((di + 1) * 9 + (dj + 1) * 3 + (dk + 1)) * n + r];
+    val[k] = v;
+    val32[k] = (float)v;
   }
 }
 
@@ -297,12 +294,14 @@ inline dim3 grid_of(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
-dErr dohpMatBuildStencil_Private(_p_dohpMat *A, dohpFS fs, hipStream_t st)
+dErr dohpMatPrepareStencil_Private(_p_dohpMat *A, dohpFS fs, bool *box, hipStream_t st)
 {
+  (void)st;
   const long L0 = (long)fs->p * fs->M[0] + 1, L1 = (long)fs->p * fs->M[1] + 1, L2 = (long)fs->p * fs->M[2] + 1;
   const int m = fs->dirichlet_mask;
   Box b{(int)(L0 - ((m & 1) ? 2 : 0)), (int)(L1 - ((m & 1) ? 2 : 0)), (int)(L2 - ((m & 2) ? 1 : 0) - ((m & 4) ? 1 : 0))};
-  if (b.n() != A->nrows || b.nx <= 0 || b.ny <= 0 || b.nz <= 0) return 0;  // not a box: CSR only
+  *box = false;
+  if (b.n() != A->nrows || b.nx <= 0 || b.ny <= 0 || b.nz <= 0) return 0;
   A->box[0] = b.nx;
   A->box[1] = b.ny;
   A->box[2] = b.nz;
@@ -310,8 +309,15 @@ dErr dohpMatBuildStencil_Private(_p_dohpMat *A, dohpFS fs, hipStream_t st)
     DOHP_CALL(A->st64.alloc(27 * b.n()));
     DOHP_CALL(A->st32.alloc(27 * b.n()));
   }
-  hipLaunchKernelGGL(csr_to_stencil_kernel, grid_of(b.n()), dim3(256), 0, st, b, A->rowptr.ptr, A->col.ptr,
-                     A->val.ptr, A->st64.ptr, A->st32.ptr);
+  *box = true;
+  return 0;
+}
+
+dErr dohpMatStencilToCSR_Private(_p_dohpMat *A, hipStream_t st)
+{
+  const Box b{A->box[0], A->box[1], A->box[2]};
+  hipLaunchKernelGGL(stencil_to_csr_kernel, grid_of(b.n()), dim3(256), 0, st, b, A->rowptr.ptr, A->col.ptr,
+                     A->st64.ptr, A->val.ptr, A->val32.ptr, A->st32.ptr);
   DOHP_HIP(hipGetLastError());
   return 0;
 }

@@ -92,7 +92,8 @@ __device__ inline int find_col(const int *__restrict__ col, int lo, int hi, int 
 __global__ void q1_assemble_kernel(int nelem, const int *__restrict__ enodes, const int *__restrict__ everts,
                                    const double *__restrict__ vxyz, const double *__restrict__ kv,
                                    const double *__restrict__ ustate, long nowned, const int *__restrict__ rowptr,
-                                   const int *__restrict__ colidx, double *__restrict__ val, Q1Tab tb, Q1Phys ph)
+                                   const int *__restrict__ colidx, double *__restrict__ val, double *__restrict__ st,
+                                   Q1Tab tb, Q1Phys ph)
 {
   const int p = tb.p, n = tb.n, nq = tb.nq, np3 = p * p * p;
   const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -231,11 +232,20 @@ __global__ void q1_assemble_kernel(int nelem, const int *__restrict__ enodes, co
   for (int r = 0; r < 8; r++) {
     const int gr = gidx[r];
     if (gr >= nowned) continue;
-    const int lo = rowptr[gr], hi = rowptr[gr + 1];
-    for (int c = 0; c < 8; c++) {
-      const int gc = gidx[c];
-      if (gc >= nowned) continue;
-      atomicAdd(&val[find_col(colidx, lo, hi, gc)], K[r][c]);
+    if (st) {  // stencil form [27][rows]: the entry's slot is the lattice offset from the row node, no search
+      for (int c = 0; c < 8; c++) {
+        const int gc = gidx[c];
+        if (gc >= nowned) continue;
+        const int sl = ((c >> 2) - (r >> 2) + 1) * 9 + (((c >> 1) & 1) - ((r >> 1) & 1) + 1) * 3 + ((c & 1) - (r & 1) + 1);
+        atomicAdd(&st[sl * nowned + gr], K[r][c]);
+      }
+    } else {
+      const int lo = rowptr[gr], hi = rowptr[gr + 1];
+      for (int c = 0; c < 8; c++) {
+        const int gc = gidx[c];
+        if (gc >= nowned) continue;
+        atomicAdd(&val[find_col(colidx, lo, hi, gc)], K[r][c]);
+      }
     }
   }
 }
@@ -370,14 +380,25 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
     }
   Q1Phys ph{prm.p, prm.epsilon * prm.epsilon, prm.lambda, kvert != nullptr, fs->M[1], fs->M[2],
             fs->p * fs->M[1] + 1, fs->p * fs->M[2] + 1};
-  hipLaunchKernelGGL(zero_f64_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->nnz);
   const long nt = (long)fs->nelem * fs->p * fs->p * fs->p;
+  // On a block space the owned nodes form a box and the matrix is a 27-point stencil: the sub-cells add into the
+  // stencil form directly (coalesced atomics at computed slots), and the CSR values are read back from it.
+  bool box = false;
+  DOHP_CALL(dohpMatPrepareStencil_Private(A, fs, &box, st));
+  if (box) {
+    hipLaunchKernelGGL(zero_f64_kernel, dim3((27 * A->nrows + 255) / 256), dim3(256), 0, st, A->st64.ptr,
+                       27 * A->nrows);
+  } else {
+    hipLaunchKernelGGL(zero_f64_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->nnz);
+  }
   hipLaunchKernelGGL(q1_assemble_kernel, dim3((nt + 127) / 128), dim3(128), 0, st, fs->nelem, fs->d_elem_nodes.ptr,
                      fs->d_elem_verts.ptr, fs->d_vcoords.ptr, kvert, state, fs->nowned, A->rowptr.ptr, A->col.ptr,
-                     A->val.ptr, tb, ph);
+                     A->val.ptr, box ? A->st64.ptr : nullptr, tb, ph);
+  DOHP_HIP(hipGetLastError());
+  if (box) return dohpMatStencilToCSR_Private(A, st);
   hipLaunchKernelGGL(to_f32_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->val32.ptr, A->nnz);
   DOHP_HIP(hipGetLastError());
-  return dohpMatBuildStencil_Private(A, fs, st);
+  return 0;
 }
 
 extern "C" dErr dohpMatGetSizes(dohpMat A, long *nrows, long *nnz)

@@ -89,162 +89,195 @@ __device__ inline int find_col(const int *__restrict__ col, int lo, int hi, int 
   return lo;
 }
 
-__global__ void q1_assemble_kernel(int nelem, const int *__restrict__ enodes, const int *__restrict__ everts,
-                                   const double *__restrict__ vxyz, const double *__restrict__ kv,
-                                   const double *__restrict__ ustate, long nowned, const int *__restrict__ rowptr,
-                                   const int *__restrict__ colidx, double *__restrict__ val, double *__restrict__ st,
-                                   Q1Tab tb, Q1Phys ph)
+// position of entry (r, c) of a symmetric 8x8 block in its row-major upper triangle
+__host__ __device__ constexpr int sym_index(int r, int c)
 {
-  const int p = tb.p, n = tb.n, nq = tb.nq, np3 = p * p * p;
-  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (long)nelem * np3) return;
-  const long e = tid / np3;
-  const int pidx = (int)(tid - e * np3);
-  const int pa = pidx / (p * p), pb = (pidx / p) % p, pc = pidx % p;
-  double X[8][3], kvx[8];
-  for (int t = 0; t < 8; t++) {
-    const long vi = everts[e * 8 + t];
-    for (int d = 0; d < 3; d++) X[t][d] = vxyz[3 * vi + d];
-    kvx[t] = ph.has_k ? kv[vi] : 1.0;
-  }
-  const int *en = &enodes[e * n * n * n];
+  return r <= c ? r * 8 - r * (r - 1) / 2 + (c - r) : c * 8 - c * (c - 1) / 2 + (r - c);
+}
+
+// pow and exp out of line: their registers do not constrain the allocation of the sub-cell block
+__device__ __noinline__ double q1_pow(double x, double e) { return pow(x, e); }
+__device__ __noinline__ double q1_exp(double x) { return exp(x); }
+
+// Sub-cell kernel of the embedded-Q1 assembly for n = N nodes per direction, one thread per GLL sub-cell: the
+// 1-D tables sit in device memory (zq, wq, hsub, Bs, Ds at tab, read through the caches), the state's values at the
+// sub-cell's 8 points accumulate over the element's x planes (no per-thread copies of tables or partial sums),
+// the points' data waits in LDS and the 8x8 block in registers.
+template <int N>
+__global__ void __launch_bounds__(128) q1_cell_kernel(int nelem, const int *__restrict__ enodes,
+                                                     const int *__restrict__ everts, const double *__restrict__ vxyz,
+                                                     const double *__restrict__ kv, const double *__restrict__ ustate,
+                                                     long nowned, const int *__restrict__ rowptr,
+                                                     const int *__restrict__ colidx, double *__restrict__ val,
+                                                     double *__restrict__ st, int corrected,
+                                                     const double *__restrict__ tab, Q1Phys ph)
+{
+  constexpr int P = N - 1, NP3 = P * P * P, NQ = 2 * P;
+  // per thread: u and the reference gradient at the 8 points
+  __shared__ double PD[32][128];
+  const double *zq = tab, *wq = tab + NQ, *hsub = tab + 2 * NQ, *Bs = tab + 2 * NQ + P, *Ds = Bs + NQ * N;
+  const int lt = threadIdx.x;
+  const long tid = (long)blockIdx.x * blockDim.x + lt;
+  if (tid >= (long)nelem * NP3) return;
+  const long e = tid / NP3;
+  const int pidx = (int)(tid - e * NP3);
+  const int pa = pidx / (P * P), pb = (pidx / P) % P, pc = pidx % P;
+  const int *ev = everts + e * 8;
   const long e2 = e % ph.M2, e1 = (e / ph.M2) % ph.M1, e0 = e / ((long)ph.M1 * ph.M2);
-  // u and reference gradient at the sub-cell's 8 patch points, sum-factorized:
-  // contract c for the two kk, then b for the two jj, then a for the two ii.
-  double Tz[kMaxP + 1][kMaxP + 1][2][2];  // [a][b][kk][B|D]
-  for (int a = 0; a < n; a++)
-    for (int b = 0; b < n; b++) {
-      double s[2][2] = {{0, 0}, {0, 0}};
-      for (int c = 0; c < n; c++) {
-        const double uv = ustate[(((int)e0 * p + a) * ph.Ly + (int)e1 * p + b) * ph.Lz + (int)e2 * p + c];
-        for (int kk = 0; kk < 2; kk++) {
-          s[kk][0] += tb.Bs[2 * pc + kk][c] * uv;
-          s[kk][1] += tb.Ds[2 * pc + kk][c] * uv;
+  const long lat0 = ((e0 * P) * ph.Ly + e1 * P) * ph.Lz + e2 * P;
+  {
+    const double *bz0 = Bs + (2 * pc) * N, *bz1 = bz0 + N, *dz0 = Ds + (2 * pc) * N, *dz1 = dz0 + N;
+    const double *by0 = Bs + (2 * pb) * N, *by1 = by0 + N, *dy0 = Ds + (2 * pb) * N, *dy1 = dy0 + N;
+    const double *bx0 = Bs + (2 * pa) * N, *bx1 = bx0 + N, *dx0 = Ds + (2 * pa) * N, *dx1 = dx0 + N;
+    double PU[8], PG0[8], PG1[8], PG2[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) PU[q] = PG0[q] = PG1[q] = PG2[q] = 0.0;
+#pragma unroll 1
+    for (int a = 0; a < N; a++) {
+      double Y[2][2][3];  // [jj][kk][BB | DB | BD]
+#pragma unroll
+      for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) Y[jj][kk][0] = Y[jj][kk][1] = Y[jj][kk][2] = 0.0;
+#pragma unroll 1
+      for (int b = 0; b < N; b++) {
+        const double *ucol = ustate + lat0 + ((long)a * ph.Ly + b) * ph.Lz;
+        double s00 = 0, s01 = 0, s10 = 0, s11 = 0;  // [kk][B | D]
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const double uv = ucol[c];
+          s00 += bz0[c] * uv;
+          s01 += dz0[c] * uv;
+          s10 += bz1[c] * uv;
+          s11 += dz1[c] * uv;
+        }
+        const double B[2] = {by0[b], by1[b]}, D[2] = {dy0[b], dy1[b]};
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+          Y[jj][0][0] += B[jj] * s00;
+          Y[jj][0][1] += D[jj] * s00;
+          Y[jj][0][2] += B[jj] * s01;
+          Y[jj][1][0] += B[jj] * s10;
+          Y[jj][1][1] += D[jj] * s10;
+          Y[jj][1][2] += B[jj] * s11;
         }
       }
-      for (int kk = 0; kk < 2; kk++) {
-        Tz[a][b][kk][0] = s[kk][0];
-        Tz[a][b][kk][1] = s[kk][1];
+      const double Bx[2] = {bx0[a], bx1[a]}, Dx[2] = {dx0[a], dx1[a]};
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int ii = q >> 2, jj = (q >> 1) & 1, kk = q & 1;
+        PU[q] += Bx[ii] * Y[jj][kk][0];
+        PG0[q] += Dx[ii] * Y[jj][kk][0];
+        PG1[q] += Bx[ii] * Y[jj][kk][1];
+        PG2[q] += Bx[ii] * Y[jj][kk][2];
       }
     }
-  double Ty[kMaxP + 1][2][2][3];  // [a][jj][kk][BB | DB | BD]
-  for (int a = 0; a < n; a++)
-    for (int jj = 0; jj < 2; jj++)
-      for (int kk = 0; kk < 2; kk++) {
-        double bb = 0, db = 0, bd = 0;
-        for (int b = 0; b < n; b++) {
-          const double B = tb.Bs[2 * pb + jj][b], D = tb.Ds[2 * pb + jj][b];
-          bb += B * Tz[a][b][kk][0];
-          db += D * Tz[a][b][kk][0];
-          bd += B * Tz[a][b][kk][1];
-        }
-        Ty[a][jj][kk][0] = bb;
-        Ty[a][jj][kk][1] = db;
-        Ty[a][jj][kk][2] = bd;
-      }
-  double uq[8], gq[8][3], jwq[8], kq[8];
-  double cj[8][3][3];  // inverse Jacobian used by the sub-cell tables (element-natural index quirk)
-  for (int ii = 0; ii < 2; ii++)
-    for (int jj = 0; jj < 2; jj++)
-      for (int kk = 0; kk < 2; kk++) {
-        const int q = (ii * 2 + jj) * 2 + kk;
-        double u = 0, g0 = 0, g1 = 0, g2 = 0;
-        for (int a = 0; a < n; a++) {
-          const double B = tb.Bs[2 * pa + ii][a], D = tb.Ds[2 * pa + ii][a];
-          u += B * Ty[a][jj][kk][0];
-          g0 += D * Ty[a][jj][kk][0];
-          g1 += B * Ty[a][jj][kk][1];
-          g2 += B * Ty[a][jj][kk][2];
-        }
-        const int qi = 2 * pa + ii, qj = 2 * pb + jj, qk = 2 * pc + kk;
-        double J[3][3], Ji[3][3];
-        jac_at(X, tb.zq[qi], tb.zq[qj], tb.zq[qk], J);
-        const double det = invert3(J, Ji);
-        jwq[q] = tb.wq[qi] * tb.wq[qj] * tb.wq[qk] * det;
-        uq[q] = u;
-        // physical gradient: grad u[e] = sum_k ghat_k jinv[k][e]
-        for (int d = 0; d < 3; d++) gq[q][d] = g0 * Ji[0][d] + g1 * Ji[1][d] + g2 * Ji[2][d];
-        {
-          const double f0[2] = {0.5 * (1 - tb.zq[qi]), 0.5 * (1 + tb.zq[qi])};
-          const double f1[2] = {0.5 * (1 - tb.zq[qj]), 0.5 * (1 + tb.zq[qj])};
-          const double f2[2] = {0.5 * (1 - tb.zq[qk]), 0.5 * (1 + tb.zq[qk])};
-          double kk2 = 0;
-          for (int t = 0; t < 8; t++) kk2 += kvx[t] * f0[t >> 2] * f1[(t >> 1) & 1] * f2[t & 1];
-          kq[q] = kk2;
-        }
-        if (tb.corrected) {  // the inverse Jacobian at the point itself
-          for (int r = 0; r < 3; r++)
-            for (int d = 0; d < 3; d++) cj[q][r][d] = Ji[r][d];
-        } else {  // element-natural point pidx*8+q, decoded on the (2p)^3 sparse lattice
-          const int qe = pidx * 8 + q;
-          const int ei = qe / (nq * nq), ej = (qe / nq) % nq, ek = qe % nq;
-          double J2[3][3];
-          jac_at(X, tb.zq[ei], tb.zq[ej], tb.zq[ek], J2);
-          invert3(J2, cj[q]);
-        }
-      }
-  // sub-cell tables and the 8x8 block (ellip.c:507-520)
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      PD[q][lt] = PU[q];
+      PD[8 + q][lt] = PG0[q];
+      PD[16 + q][lt] = PG1[q];
+      PD[24 + q][lt] = PG2[q];
+    }
+  }
   const double i1[2][2] = {{0.78867513459481287, 0.21132486540518713}, {0.21132486540518713, 0.78867513459481287}};
   const double d1[2][2] = {{-1., 1.}, {-1., 1.}};
-  // per-direction derivative scale of the sub-cell's hat functions: 1 in the reference's arithmetic,
-  // 1/h (the sub-interval width) when corrected
-  const double sx = tb.corrected ? 1.0 / tb.hsub[pa] : 1.0, sy = tb.corrected ? 1.0 / tb.hsub[pb] : 1.0,
-               sz = tb.corrected ? 1.0 / tb.hsub[pc] : 1.0;
-  double K[8][8];
-  for (int r = 0; r < 8; r++)
-    for (int c = 0; c < 8; c++) K[r][c] = 0.0;
+  const double sx = corrected ? 1.0 / hsub[pa] : 1.0, sy = corrected ? 1.0 / hsub[pb] : 1.0,
+               sz = corrected ? 1.0 / hsub[pc] : 1.0;
+  // the block is symmetric (EllipPointwiseJacobian below gives K[r][c] = -w lambda e^u I_r I_c + eta w D_r.D_c
+  // - w (s.D_r)(s.D_c)): its upper triangle, row-major, is accumulated
+  double K[36];
+#pragma unroll
+  for (int t = 0; t < 36; t++) K[t] = 0.0;
+#pragma unroll 1
   for (int q = 0; q < 8; q++) {
     const int ii = q >> 2, jj = (q >> 1) & 1, kk = q & 1;
+    const int qi = 2 * pa + ii, qj = 2 * pb + jj, qk = 2 * pc + kk;
+    double X[8][3];  // (read here, through the caches, so that they are not live in the block's accumulation)
+    for (int t = 0; t < 8; t++)
+      for (int d = 0; d < 3; d++) X[t][d] = vxyz[3 * (long)ev[t] + d];
+    double J[3][3], Ji[3][3], cj[3][3];
+    jac_at(X, zq[qi], zq[qj], zq[qk], J);
+    const double det = invert3(J, Ji);
+    const double w = wq[qi] * wq[qj] * wq[qk] * det;
+    const double g0 = PD[8 + q][lt], g1 = PD[16 + q][lt], g2 = PD[24 + q][lt];
+    double Du[3];
+    for (int d = 0; d < 3; d++) Du[d] = g0 * Ji[0][d] + g1 * Ji[1][d] + g2 * Ji[2][d];
+    double kq = 0;
+    {
+      const double f0[2] = {0.5 * (1 - zq[qi]), 0.5 * (1 + zq[qi])};
+      const double f1[2] = {0.5 * (1 - zq[qj]), 0.5 * (1 + zq[qj])};
+      const double f2[2] = {0.5 * (1 - zq[qk]), 0.5 * (1 + zq[qk])};
+      for (int t = 0; t < 8; t++) kq += (ph.has_k ? kv[ev[t]] : 1.0) * f0[t >> 2] * f1[(t >> 1) & 1] * f2[t & 1];
+    }
+    if (corrected) {  // the inverse Jacobian at the point itself
+      for (int r = 0; r < 3; r++)
+        for (int d = 0; d < 3; d++) cj[r][d] = Ji[r][d];
+    } else {  // element-natural point pidx*8+q, decoded on the (2p)^3 sparse lattice
+      const int qe = pidx * 8 + q;
+      const int ei = qe / (NQ * NQ), ej = (qe / NQ) % NQ, ek = qe % NQ;
+      double J2[3][3];
+      jac_at(X, zq[ei], zq[ej], zq[ek], J2);
+      invert3(J2, cj);
+    }
     // EllipPointwiseComputeStore (ellip.c:333-343) with the coefficient factor
-    const double *Du = gq[q];
     const double gamma = 0.5 * (Du[0] * Du[0] + Du[1] * Du[1] + Du[2] * Du[2]);
     const double espg = ph.eps2 + gamma;
-    double eta = pow(espg, (ph.p - 2) / 2);
-    const double smdc = sqrt(-((ph.p - 2) / 2) * eta / espg) * sqrt(kq[q]);
-    eta *= kq[q];
+    double eta = q1_pow(espg, (ph.p - 2) / 2);
+    const double smdc = sqrt(-((ph.p - 2) / 2) * eta / espg) * sqrt(kq);
+    eta *= kq;
     const double smd[3] = {smdc * Du[0], smdc * Du[1], smdc * Du[2]};
-    const double lexp = ph.lambda != 0 ? ph.lambda * exp(uq[q]) : 0.0;
-    const double w = jwq[q];
+    const double lexp = ph.lambda != 0 ? ph.lambda * q1_exp(PD[q][lt]) : 0.0;
     double I8[8], D8[8][3];
+#pragma unroll
     for (int b = 0; b < 8; b++) {
       const int a0 = b >> 2, a1 = (b >> 1) & 1, a2 = b & 1;
       const double dX = sx * d1[ii][a0] * i1[jj][a1] * i1[kk][a2];
       const double dY = sy * i1[ii][a0] * d1[jj][a1] * i1[kk][a2];
       const double dZ = sz * i1[ii][a0] * i1[jj][a1] * d1[kk][a2];
       I8[b] = i1[ii][a0] * i1[jj][a1] * i1[kk][a2];
-      for (int d = 0; d < 3; d++) D8[b][d] = dX * cj[q][0][d] + dY * cj[q][1][d] + dZ * cj[q][2][d];
+#pragma unroll
+      for (int d = 0; d < 3; d++) D8[b][d] = dX * cj[0][d] + dY * cj[1][d] + dZ * cj[2][d];
     }
-    for (int c = 0; c < 8; c++) {
-      // EllipPointwiseJacobian (ellip.c:369-377) on basis function c
-      const double dotw = (smd[0] * D8[c][0] + smd[1] * D8[c][1] + smd[2] * D8[c][2]) * w;
-      const double etaw = eta * w;
-      const double v = -w * lexp * I8[c];
-      double dv[3];
-      for (int d = 0; d < 3; d++) dv[d] = etaw * D8[c][d] - dotw * smd[d];
-      for (int r = 0; r < 8; r++) K[r][c] += I8[r] * v + D8[r][0] * dv[0] + D8[r][1] * dv[1] + D8[r][2] * dv[2];
-    }
+    // EllipPointwiseJacobian (ellip.c:369-377) on basis function c, tested against basis function r:
+    // v_c = -w lambda e^u I_c, Dv_c = eta w D_c - w (s.D_c) s
+    const double etaw = eta * w, lw = -w * lexp;
+    double sd[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) sd[b] = smd[0] * D8[b][0] + smd[1] * D8[b][1] + smd[2] * D8[b][2];
+#pragma unroll
+    for (int r = 0, t = 0; r < 8; r++)
+#pragma unroll
+      for (int c = r; c < 8; c++, t++)
+        K[t] += I8[r] * (lw * I8[c]) + etaw * (D8[r][0] * D8[c][0] + D8[r][1] * D8[c][1] + D8[r][2] * D8[c][2]) -
+                w * sd[r] * sd[c];
   }
+  const int *en = &enodes[e * N * N * N];
   int gidx[8];
+#pragma unroll
   for (int b = 0; b < 8; b++) {
     const int a0 = b >> 2, a1 = (b >> 1) & 1, a2 = b & 1;
-    gidx[b] = en[((pa + a0) * n + pb + a1) * n + pc + a2];
+    gidx[b] = en[((pa + a0) * N + pb + a1) * N + pc + a2];
   }
+#pragma unroll
   for (int r = 0; r < 8; r++) {
     const int gr = gidx[r];
     if (gr >= nowned) continue;
-    if (st) {  // stencil form [27][rows]: the entry's slot is the lattice offset from the row node, no search
+    if (st) {  // stencil form [27][rows]: the entry's slot is the lattice offset from the row node
+#pragma unroll
       for (int c = 0; c < 8; c++) {
         const int gc = gidx[c];
         if (gc >= nowned) continue;
         const int sl = ((c >> 2) - (r >> 2) + 1) * 9 + (((c >> 1) & 1) - ((r >> 1) & 1) + 1) * 3 + ((c & 1) - (r & 1) + 1);
-        atomicAdd(&st[sl * nowned + gr], K[r][c]);
+        atomicAdd(&st[sl * nowned + gr], K[sym_index(r, c)]);
       }
     } else {
       const int lo = rowptr[gr], hi = rowptr[gr + 1];
+#pragma unroll
       for (int c = 0; c < 8; c++) {
         const int gc = gidx[c];
         if (gc >= nowned) continue;
-        atomicAdd(&val[find_col(colidx, lo, hi, gc)], K[r][c]);
+        atomicAdd(&val[find_col(colidx, lo, hi, gc)], K[sym_index(r, c)]);
       }
     }
   }
@@ -391,9 +424,40 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   } else {
     hipLaunchKernelGGL(zero_f64_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->nnz);
   }
-  hipLaunchKernelGGL(q1_assemble_kernel, dim3((nt + 127) / 128), dim3(128), 0, st, fs->nelem, fs->d_elem_nodes.ptr,
-                     fs->d_elem_verts.ptr, fs->d_vcoords.ptr, kvert, state, fs->nowned, A->rowptr.ptr, A->col.ptr,
-                     A->val.ptr, box ? A->st64.ptr : nullptr, tb, ph);
+  {  // 1-D tables for the kernel: zq, wq (2p each), hsub (p), Bs, Ds (2p x n each)
+    const int nq2 = tb.nq, pp = tb.p, nn = tb.n;
+    std::vector<double> tab;
+    tab.insert(tab.end(), tb.zq, tb.zq + nq2);
+    tab.insert(tab.end(), tb.wq, tb.wq + nq2);
+    tab.insert(tab.end(), tb.hsub, tb.hsub + pp);
+    for (int q = 0; q < nq2; q++) tab.insert(tab.end(), tb.Bs[q], tb.Bs[q] + nn);
+    for (int q = 0; q < nq2; q++) tab.insert(tab.end(), tb.Ds[q], tb.Ds[q] + nn);
+    if (A->q1tab.n != tab.size()) DOHP_CALL(A->q1tab.alloc(tab.size()));
+    DOHP_HIP(hipMemcpyAsync(A->q1tab.ptr, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, st));
+    DOHP_HIP(hipStreamSynchronize(st));  // the host vector goes out of scope
+  }
+  const dim3 grid((unsigned)((nt + 127) / 128));
+  double *stp = box ? A->st64.ptr : nullptr;
+  switch (fs->n) {
+#define DOHP_CASE(NN)                                                                                              \
+  case NN:                                                                                                         \
+    hipLaunchKernelGGL(q1_cell_kernel<NN>, grid, dim3(128), 0, st, fs->nelem, fs->d_elem_nodes.ptr,               \
+                       fs->d_elem_verts.ptr, fs->d_vcoords.ptr, kvert, state, fs->nowned, A->rowptr.ptr, A->col.ptr, \
+                       A->val.ptr, stp, tb.corrected, A->q1tab.ptr, ph);                                          \
+    break;
+    DOHP_CASE(2)
+    DOHP_CASE(3)
+    DOHP_CASE(4)
+    DOHP_CASE(5)
+    DOHP_CASE(6)
+    DOHP_CASE(7)
+    DOHP_CASE(8)
+    DOHP_CASE(9)
+    DOHP_CASE(10)
+    DOHP_CASE(11)
+#undef DOHP_CASE
+  default: set_error("embedded-Q1 assembly supports degree 1..10"); return dohpERR_ARG;
+  }
   DOHP_HIP(hipGetLastError());
   if (box) return dohpMatStencilToCSR_Private(A, st);
   hipLaunchKernelGGL(to_f32_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->val32.ptr, A->nnz);

@@ -326,8 +326,60 @@ extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpElli
 // Pattern of the Q1 matrix on the GLL sub-cell lattice: row of every owned node, columns of the
 // owned nodes among its 27 lattice neighbours (dFSCreateMatrix preallocates 27 per row,
 // src/fs/interface/fs.c:770-813).
+// CSR pattern of the 27-point stencil on a box of rows in lexicographic order, built on the device: row (i, j, k)
+// couples to its box neighbours in increasing index, and its offset has a closed form (the count of entries of the
+// rows before it, a product of per-direction counts 1 + [i > 0] + [i < n - 1]).
+__device__ __forceinline__ long box_prefix(int i, int n) { return i + (i > 0 ? i - 1 : 0) + (i < n - 1 ? i : n - 1); }
+
+__global__ void box_pattern_kernel(int nx, int ny, int nz, int *__restrict__ rowptr, int *__restrict__ col)
+{
+  const long n = (long)nx * ny * nz;
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int k = (int)(r % nz), j = (int)((r / nz) % ny), i = (int)(r / ((long)ny * nz));
+  const long cj = 1 + (j > 0) + (j < ny - 1), ck = 1 + (k > 0) + (k < nz - 1), ci = 1 + (i > 0) + (i < nx - 1);
+  const long Sy = 3L * ny - 2, Sz = 3L * nz - 2;
+  const long off = box_prefix(i, nx) * Sy * Sz + ci * box_prefix(j, ny) * Sz + ci * cj * box_prefix(k, nz);
+  rowptr[r] = (int)off;
+  if (r == n - 1) rowptr[n] = (int)((3L * nx - 2) * Sy * Sz);
+  long m = off;
+  for (int di = -1; di <= 1; di++)
+    for (int dj = -1; dj <= 1; dj++)
+      for (int dk = -1; dk <= 1; dk++) {
+        const int a = i + di, b = j + dj, c = k + dk;
+        if (a < 0 || b < 0 || c < 0 || a >= nx || b >= ny || c >= nz) continue;
+        col[m++] = (int)(((long)a * ny + b) * nz + c);
+      }
+}
+
+static dErr build_pattern_box(dohpFS fs, _p_dohpMat *A, bool *done)
+{
+  const long L0 = (long)fs->p * fs->M[0] + 1, L1 = (long)fs->p * fs->M[1] + 1, L2 = (long)fs->p * fs->M[2] + 1;
+  const int m = fs->dirichlet_mask;
+  const long nx = L0 - ((m & 1) ? 2 : 0), ny = L1 - ((m & 1) ? 2 : 0), nz = L2 - ((m & 2) ? 1 : 0) - ((m & 4) ? 1 : 0);
+  *done = false;
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nx * ny * nz != fs->nowned) return 0;
+  const long nnz = (3 * nx - 2) * (3 * ny - 2) * (3 * nz - 2);
+  if (nnz > 2147483647L) return 0;  // 32-bit offsets
+  A->nrows = fs->nowned;
+  A->nnz = nnz;
+  DOHP_CALL(A->rowptr.alloc(A->nrows + 1));
+  DOHP_CALL(A->col.alloc(nnz));
+  DOHP_CALL(A->val.alloc(nnz));
+  DOHP_CALL(A->val32.alloc(nnz));
+  hipLaunchKernelGGL(box_pattern_kernel, dim3((unsigned)((A->nrows + 255) / 256)), dim3(256), 0, nullptr, (int)nx,
+                     (int)ny, (int)nz, A->rowptr.ptr, A->col.ptr);
+  DOHP_HIP(hipGetLastError());
+  DOHP_HIP(hipDeviceSynchronize());
+  *done = true;
+  return 0;
+}
+
 static dErr build_pattern(dohpFS fs, _p_dohpMat *A)
 {
+  bool done = false;
+  DOHP_CALL(build_pattern_box(fs, A, &done));
+  if (done) return 0;
   const long L0 = (long)fs->p * fs->M[0] + 1, L1 = (long)fs->p * fs->M[1] + 1, L2 = (long)fs->p * fs->M[2] + 1;
   // invert the closure numbering: lattice index of each closure node
   std::vector<long> c2l(fs->nclosure);

@@ -323,9 +323,6 @@ __global__ void zero_f64_kernel(double *__restrict__ a, long n)
 extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpEllipParam *prm, const double **kvert,
                                               const double **state, int *q1mode);
 
-// Pattern of the Q1 matrix on the GLL sub-cell lattice: row of every owned node, columns of the
-// owned nodes among its 27 lattice neighbours (dFSCreateMatrix preallocates 27 per row,
-// src/fs/interface/fs.c:770-813).
 // CSR pattern of the 27-point stencil on a box of rows in lexicographic order, built on the device: row (i, j, k)
 // couples to its box neighbours in increasing index, and its offset has a closed form (the count of entries of the
 // rows before it, a product of per-direction counts 1 + [i > 0] + [i < n - 1]).
@@ -375,6 +372,9 @@ static dErr build_pattern_box(dohpFS fs, _p_dohpMat *A, bool *done)
   return 0;
 }
 
+// Pattern of the Q1 matrix on the GLL sub-cell lattice: row of every owned node, columns of the
+// owned nodes among its 27 lattice neighbours (dFSCreateMatrix preallocates 27 per row,
+// src/fs/interface/fs.c:770-813).
 static dErr build_pattern(dohpFS fs, _p_dohpMat *A)
 {
   bool done = false;

@@ -119,6 +119,9 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
   // Every slot's indices first, then every load (slots without a value read entry 0 and drop it: no
   // branch separates the loads, so they are in flight together), then the tile.
   T pre2[MODE == MODE_JAC ? NS : 1];
+  // stored-data Jacobian: the tensor of this thread's line, loaded with the gathered values (before
+  // the barrier) so that both are in flight together
+  T gq[MODE == MODE_JQD ? 6 : 1][MODE == MODE_JQD ? N : 1];
   {
     constexpr bool NEED_U = MODE == MODE_FUN || MODE == MODE_JAC || MODE == MODE_QD;
     constexpr bool NEED_X = MODE == MODE_LIN || MODE == MODE_JAC || MODE == MODE_JQD;
@@ -148,6 +151,16 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
 #pragma unroll
       for (int s = 0; s < NS; s++) uv[s] = ustate[lat[s] > 0 ? lat[s] : 0];
     }
+    if constexpr (MODE == MODE_JQD) {  // line and element as in the body; an empty slot reads element 0
+      const int ln = tid % (EPB * NN), l = ln / NN;
+      const long es = lb * EPB + l;
+      const long el = es < ne ? (elist ? (long)elist[es] : es) : 0;
+      const T *q = ustate + el * (ph.lambda != 0 ? 7 : 6) * N3 + (ln - l * NN);
+#pragma unroll
+      for (int k = 0; k < N; k++)
+#pragma unroll
+        for (int c = 0; c < 6; c++) gq[c][k] = q[c * N3 + k * NN];
+    }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       const int t = s * NT + tid;
@@ -172,7 +185,9 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
     else                     \
       y[c] += val;           \
   } while (0)
+#define DOHP_QD_PRELOADED
 #include "ellip_batch.inc"
+#undef DOHP_QD_PRELOADED
 #undef DOHP_SCATTER
 #undef DOHP_BATCH_HOOK
 #undef DOHP_ELEM

@@ -104,6 +104,11 @@ class MG:
     def apply_f32(self, r, z, stream=None):
         check(_lib.dohpMGApplyF32(self.h, _ptr(r), _ptr(z), _stream(stream)))
 
+    def residual(self, level, b, x, r, stream=None):
+        """r = b - A_level x (b None: r = -A x); FP32 tensors use the level's FP32 coefficients."""
+        f = _lib.dohpMGResidualF32 if x.element_size() == 4 else _lib.dohpMGResidual
+        check(f(self.h, int(level), None if b is None else _ptr(b), _ptr(x), _ptr(r), _stream(stream)))
+
     def __del__(self):
         if _lib is not None and getattr(self, "h", None):
             _lib.dohpMGDestroy(ctypes.byref(self.h))
@@ -127,6 +132,8 @@ _proto("dohpMGGetInfo", _vp, _vp, _vp, _vp)
 _proto("dohpMGGetLevel", _vp, ctypes.c_int, _vp, _vp)
 _proto("dohpMGApply", _vp, _vp, _vp, _vp)
 _proto("dohpMGApplyF32", _vp, _vp, _vp, _vp)
+_proto("dohpMGResidual", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)
+_proto("dohpMGResidualF32", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)
 _proto("dohpMGDestroy", _vp)
 _proto("dohpEllipSolve", _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)
 _proto("dohpEllipSolveLearned", _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)

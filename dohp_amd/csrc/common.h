@@ -96,6 +96,14 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// Logical block index: consecutive workgroups are dealt round-robin to the eight XCDs, so
+// give each XCD a contiguous range of work (neighbours share data through its L2).
+__device__ inline long xcd_block(long b, long nb)
+{
+  const long q = nb / 8, r = nb % 8, x = b % 8, pos = b / 8;
+  return x < r ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
+}
+
 }  // namespace dohp
 
 // Internal entry points shared between the operator and the Q1 assembly.

@@ -67,14 +67,6 @@ __device__ double forcing(const Phys &ph, double x, double y, double z)
   return -eta * (H[0] + H[4] + H[8]) - (ph.p - 2) / 2 * pow(e2g, (ph.p - 4) / 2) * quad;
 }
 
-// Logical block index: consecutive workgroups are dealt round-robin to the eight XCDs, so
-// give each XCD a contiguous range of elements (neighbours share nodes through its L2).
-__device__ inline long xcd_block(long b, long nb)
-{
-  const long q = nb / 8, r = nb % 8, x = b % 8, pos = b / 8;
-  return x < r ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
-}
-
 // Polynomial form of the column geometry: with c0, c1 linear in g = (1+z_k)/2 and c2 constant,
 // r0 = c1 x c2 and r1 = c2 x c0 are linear in g, r2 = c0 x c1 and det = c0 . r0 quadratic.  The
 // coefficients are formed in FP64 from the vertex coordinates, once per column; evaluating them

@@ -119,6 +119,13 @@ __global__ void __launch_bounds__(256) half_residual_kernel(Box bx, const T *__r
   r[i] = (b ? b[i] : T(0)) - acc;
 }
 
+// r = b - A x (b may be null) on a level.
+template <typename T>
+void residual(const Box &bx, const T *H, const T *b, const T *x, T *r, hipStream_t st)
+{
+  hipLaunchKernelGGL(half_residual_kernel<T>, box_grid(bx), box_block(), 0, st, bx, H, b, x, r);
+}
+
 __global__ void half_from_full_kernel(long n, const double *__restrict__ A, double *__restrict__ H,
                                       float *__restrict__ H32)
 {
@@ -387,8 +394,7 @@ dErr estimate_lmax(MGLevel &lv, hipStream_t st, int its)
   DOHP_CALL(norm(v.ptr, &nv));
   hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0 / nv, nullptr, v.ptr, v.ptr);
   for (int k = 0; k < its; k++) {
-    hipLaunchKernelGGL(half_residual_kernel<double>, box_grid(lv.box), box_block(), 0, st, lv.box, lv.H64.ptr,
-                       (const double *)nullptr, v.ptr, w.ptr);  // w = -A v
+    residual<double>(lv.box, lv.H64.ptr, nullptr, v.ptr, w.ptr, st);  // w = -A v
     hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, -1.0, lv.dinv64.ptr, w.ptr, w.ptr);
     DOHP_CALL(norm(w.ptr, &nw));
     hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0 / nw, nullptr, w.ptr, v.ptr);
@@ -475,11 +481,11 @@ void smooth(const _p_dohpMG *mg, MGLevel &lv, const T *b, T *x, bool zero_init, 
   if (zero_init)
     hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, T(0), (T)(1.0 / theta), P::dinv(lv), b, d, x, 1);
   else {
-    hipLaunchKernelGGL(half_residual_kernel<T>, box_grid(lv.box), box_block(), 0, st, lv.box, P::A(lv), b, x, r);
+    residual<T>(lv.box, P::A(lv), b, x, r, st);
     hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, T(0), (T)(1.0 / theta), P::dinv(lv), r, d, x, 0);
   }
   for (int k = 1; k < mg->degree; k++) {
-    hipLaunchKernelGGL(half_residual_kernel<T>, box_grid(lv.box), box_block(), 0, st, lv.box, P::A(lv), b, x, r);
+    residual<T>(lv.box, P::A(lv), b, x, r, st);
     const double rho_new = 1.0 / (2.0 * sigma - rho);
     hipLaunchKernelGGL(cheb_step_kernel<T>, g, dim3(256), 0, st, lv.n, (T)(rho_new * rho), (T)(2.0 * rho_new / delta),
                        P::dinv(lv), r, d, x, 0);
@@ -499,7 +505,7 @@ void vcycle(_p_dohpMG *mg, size_t l, const T *b, T *x, hipStream_t st)
   }
   MGLevel &cv = mg->L[l + 1];
   smooth<T>(mg, lv, b, x, true, st);
-  hipLaunchKernelGGL(half_residual_kernel<T>, box_grid(lv.box), box_block(), 0, st, lv.box, P::A(lv), b, x, P::r(lv));
+  residual<T>(lv.box, P::A(lv), b, x, P::r(lv), st);
   hipLaunchKernelGGL(restrict_kernel<T>, box_grid(cv.box), box_block(), 0, st, lv.box, cv.box, P::r(lv), P::b(cv));
   vcycle<T>(mg, l + 1, P::b(cv), P::x(cv), st);
   hipLaunchKernelGGL(prolong_add_kernel<T>, box_grid(lv.box), box_block(), 0, st, lv.box, cv.box, P::x(cv), x);
@@ -617,6 +623,26 @@ extern "C" dErr dohpMGApplyF32(dohpMG mg, const float *r, float *z, void *stream
 {
   DOHP_CHECK(mg && r && z, dohpERR_ARG, "null argument");
   vcycle<float>(mg, 0, r, z, (hipStream_t)stream);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" dErr dohpMGResidual(dohpMG mg, int level, const double *b, const double *x, double *r, void *stream)
+{
+  DOHP_CHECK(mg && level >= 0 && level < (int)mg->L.size(), dohpERR_ARG, "no such multigrid level");
+  DOHP_CHECK(x && r, dohpERR_ARG, "null argument");
+  const MGLevel &lv = mg->L[level];
+  residual<double>(lv.box, lv.H64.ptr, b, x, r, (hipStream_t)stream);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" dErr dohpMGResidualF32(dohpMG mg, int level, const float *b, const float *x, float *r, void *stream)
+{
+  DOHP_CHECK(mg && level >= 0 && level < (int)mg->L.size(), dohpERR_ARG, "no such multigrid level");
+  DOHP_CHECK(x && r, dohpERR_ARG, "null argument");
+  const MGLevel &lv = mg->L[level];
+  residual<float>(lv.box, lv.H32.ptr, b, x, r, (hipStream_t)stream);
   DOHP_HIP(hipGetLastError());
   return 0;
 }

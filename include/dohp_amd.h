@@ -226,6 +226,11 @@ dErr dohpMGGetInfo(dohpMG mg, int *nlevels, long *ncoarse, double *lmax_fine);
 dErr dohpMGGetLevel(dohpMG mg, int level, int *box, double *stencil_host);
 dErr dohpMGApply(dohpMG mg, const double *r_dev, double *z_dev, void *stream);
 dErr dohpMGApplyF32(dohpMG mg, const float *r_dev, float *z_dev, void *stream);
+/* r = b - A_l x on level `level` (b may be NULL: r = -A_l x), the residual the V-cycle's smoothers
+ * and restriction use (in FP32 with the level's FP32 coefficients); the counterpart of PETSc's
+ * MatResidual on the level operator.  r must not alias b or x. */
+dErr dohpMGResidual(dohpMG mg, int level, const double *b_dev, const double *x_dev, double *r_dev, void *stream);
+dErr dohpMGResidualF32(dohpMG mg, int level, const float *b_dev, const float *x_dev, float *r_dev, void *stream);
 dErr dohpMGDestroy(dohpMG *mg);
 
 /* ------------------------------------------- learned correction of the Q1 preconditioner */

@@ -93,6 +93,34 @@ def test_multigrid_galerkin_levels_and_symmetry(C):
     assert float(torch.dot(z1, r1)) > 0
 
 
+@pytest.mark.parametrize("M,deg", [((6, 7, 5), 5), ((3, 4, 14), 5)])
+def test_level_residual_matches_the_stencil_product(C, M, deg):
+    """The multigrid's residual r = b - A_l x on the stored half stencil against the level's 27-point stencil as
+    a CSR product, on every level of boxes whose dimensions are not multiples of the launch tiles (29 x 34 x 24
+    and 14 x 19 x 69 owned nodes at the finest level), with and without b, in FP64 and with the level's FP32
+    coefficients."""
+    print(f"level residual M={M} p={deg}", flush=True)
+    fs, op, q1, xs, b = _problem(C, M, deg)
+    mg = C.MG(q1, smooth_degree=2, max_coarse=64)
+    assert mg.level(0)[0] == tuple(m * deg - 1 for m in M)
+    for l in range(mg.nlevels):
+        box, st = mg.level(l)
+        A = _stencil_to_csr(box, st)
+        n = A.shape[0]
+        x = torch.as_tensor(splitmix64_uniform(11 + l, n) - 0.5, dtype=torch.float64, device="cuda")
+        bb = torch.as_tensor(splitmix64_uniform(31 + l, n) - 0.5, dtype=torch.float64, device="cuda")
+        xh, bh = x.cpu().numpy(), bb.cpu().numpy()
+        scale = (abs(A) @ np.abs(xh)).max()
+        r = torch.full_like(x, np.nan)
+        mg.residual(l, bb, x, r)
+        np.testing.assert_allclose(r.cpu().numpy(), bh - A @ xh, rtol=0, atol=1e-14 * scale)
+        mg.residual(l, None, x, r)
+        np.testing.assert_allclose(r.cpu().numpy(), -(A @ xh), rtol=0, atol=1e-14 * scale)
+        r32 = torch.full_like(x, np.nan, dtype=torch.float32)
+        mg.residual(l, bb.float(), x.float(), r32)
+        np.testing.assert_allclose(r32.cpu().numpy(), bh - A @ xh, rtol=0, atol=2e-6 * scale)
+
+
 @pytest.mark.parametrize("M,deg", [((6, 6, 6), 5), ((4, 5, 3), 3)])
 def test_solve_plain_and_mixed_reach_1e10(C, M, deg):
     print(f"solve M={M} p={deg}", flush=True)

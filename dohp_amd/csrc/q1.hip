@@ -311,6 +311,12 @@ __global__ void to_f32_kernel(const double *__restrict__ a, float *__restrict__ 
   if (i < n) b[i] = (float)a[i];
 }
 
+__global__ void copy_f64_kernel(const double *__restrict__ a, double *__restrict__ b, long n)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 __global__ void zero_f64_kernel(double *__restrict__ a, long n)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -558,8 +564,12 @@ extern "C" dErr dohpMatMultF32(dohpMat A, const float *x, float *y, void *stream
 extern "C" dErr dohpMatGetDiagonal(dohpMat A, double *d, void *stream)
 {
   DOHP_CHECK(A && d, dohpERR_ARG, "null argument");
-  hipLaunchKernelGGL(csr_diag_kernel, dim3((A->nrows + 255) / 256), dim3(256), 0, (hipStream_t)stream, A->nrows,
-                     A->rowptr.ptr, A->col.ptr, A->val.ptr, d);
+  if (A->st64.ptr)  // the stencil form holds the diagonal as its middle plane (the CSR values are read from it)
+    hipLaunchKernelGGL(copy_f64_kernel, dim3((A->nrows + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       A->st64.ptr + 13 * A->nrows, d, A->nrows);
+  else
+    hipLaunchKernelGGL(csr_diag_kernel, dim3((A->nrows + 255) / 256), dim3(256), 0, (hipStream_t)stream, A->nrows,
+                       A->rowptr.ptr, A->col.ptr, A->val.ptr, d);
   DOHP_HIP(hipGetLastError());
   return 0;
 }

@@ -63,7 +63,26 @@ __global__ void stencil_to_csr_kernel(Box b, const int *__restrict__ rowptr, con
   for (int s = 0; s < 27; s++) st32[s * n + r] = (float)st[s * n + r];
   int I, J, K;
   unrank(b, r, I, J, K);
-  for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
+  // the box pattern: the row's columns are its in-box neighbours in stencil order (checked column by
+  // column; a row with any other pattern takes the general path below)
+  const long sy = b.nz, sx = (long)b.ny * b.nz;
+  const int k0 = rowptr[r], k1 = rowptr[r + 1];
+  int k = k0;
+  bool box = true;
+  for (int s = 0; s < 27 && box; s++) {
+    const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
+    if (I + di < 0 || I + di >= b.nx || J + dj < 0 || J + dj >= b.ny || K + dk < 0 || K + dk >= b.nz) continue;
+    if (k < k1 && col[k] == r + di * sx + dj * sy + dk) {
+      const double v = st[s * n + r];
+      val[k] = v;
+      val32[k] = (float)v;
+      k++;
+    } else {
+      box = false;
+    }
+  }
+  if (box && k == k1) return;
+  for (int k = k0; k < k1; k++) {
     int I2, J2, K2;
     unrank(b, col[k], I2, J2, K2);
     const int di = I2 - I, dj = J2 - J, dk = K2 - K;

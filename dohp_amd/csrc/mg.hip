@@ -52,42 +52,43 @@ __device__ __forceinline__ void unrank(const Box &b, long r, int &I, int &J, int
   K = (int)(rem - (long)J * b.nz);
 }
 
-// stencil -> CSR values (and the FP32 copies of both).
-__global__ void stencil_to_csr_kernel(Box b, const int *__restrict__ rowptr, const int *__restrict__ col,
-                                      const double *__restrict__ st, double *__restrict__ val,
-                                      float *__restrict__ val32, float *__restrict__ st32)
+// stencil -> CSR values (and the FP32 copies of both).  A workgroup takes 256 consecutive rows: it
+// converts their stencil planes to FP32 (one row per thread) and walks their CSR entries, which are
+// contiguous, one entry per thread (coalesced), finding each entry's row by binary search in the
+// rows' offsets staged in LDS.
+__global__ void __launch_bounds__(256) stencil_to_csr_kernel(Box b, const int *__restrict__ rowptr,
+                                                             const int *__restrict__ col,
+                                                             const double *__restrict__ st, double *__restrict__ val,
+                                                             float *__restrict__ val32, float *__restrict__ st32)
 {
+  __shared__ int rp[257];
   const long n = b.n();
-  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  for (int s = 0; s < 27; s++) st32[s * n + r] = (float)st[s * n + r];
-  int I, J, K;
-  unrank(b, r, I, J, K);
-  // the box pattern: the row's columns are its in-box neighbours in stencil order (checked column by
-  // column; a row with any other pattern takes the general path below)
-  const long sy = b.nz, sx = (long)b.ny * b.nz;
-  const int k0 = rowptr[r], k1 = rowptr[r + 1];
-  int k = k0;
-  bool box = true;
-  for (int s = 0; s < 27 && box; s++) {
-    const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
-    if (I + di < 0 || I + di >= b.nx || J + dj < 0 || J + dj >= b.ny || K + dk < 0 || K + dk >= b.nz) continue;
-    if (k < k1 && col[k] == r + di * sx + dj * sy + dk) {
-      const double v = st[s * n + r];
-      val[k] = v;
-      val32[k] = (float)v;
-      k++;
-    } else {
-      box = false;
-    }
+  const long r0 = (long)blockIdx.x * 256;
+  const int nr = (int)min(256L, n - r0), t = threadIdx.x;
+  if (t < nr) {
+    const long r = r0 + t;
+    for (int s = 0; s < 27; s++) st32[s * n + r] = (float)st[s * n + r];
   }
-  if (box && k == k1) return;
-  for (int k = k0; k < k1; k++) {
-    int I2, J2, K2;
+  for (int j = t; j <= nr; j += 256) rp[j] = rowptr[r0 + j];
+  __syncthreads();
+  const int kb = rp[0], ke = rp[nr];
+  for (int k = kb + t; k < ke; k += 256) {
+    int lo = 0, hi = nr - 1;  // last row whose entries start at or before k
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rp[mid] <= k)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const long r = r0 + lo;
+    int I, J, K, I2, J2, K2;
+    unrank(b, r, I, J, K);
     unrank(b, col[k], I2, J2, K2);
     const int di = I2 - I, dj = J2 - J, dk = K2 - K;
     double v = 0.0;
-    if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1) v = st[((di + 1) * 9 + (dj + 1) * 3 + (dk + 1)) * n + r];
+    if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1)
+      v = st[((di + 1) * 9 + (dj + 1) * 3 + (dk + 1)) * n + r];
     val[k] = v;
     val32[k] = (float)v;
   }
@@ -342,8 +343,8 @@ dErr dohpMatPrepareStencil_Private(_p_dohpMat *A, dohpFS fs, bool *box, hipStrea
 dErr dohpMatStencilToCSR_Private(_p_dohpMat *A, hipStream_t st)
 {
   const Box b{A->box[0], A->box[1], A->box[2]};
-  hipLaunchKernelGGL(stencil_to_csr_kernel, grid_of(b.n()), dim3(256), 0, st, b, A->rowptr.ptr, A->col.ptr,
-                     A->st64.ptr, A->val.ptr, A->val32.ptr, A->st32.ptr);
+  hipLaunchKernelGGL(stencil_to_csr_kernel, dim3((unsigned)((b.n() + 255) / 256)), dim3(256), 0, st, b, A->rowptr.ptr,
+                     A->col.ptr, A->st64.ptr, A->val.ptr, A->val32.ptr, A->st32.ptr);
   DOHP_HIP(hipGetLastError());
   return 0;
 }

@@ -67,25 +67,46 @@ FP64_PEAK_TFLOPS = 78.6
 
 
 def sine_state(X, seed):
-    from dohp_amd.mesh import splitmix64_uniform
-    r = splitmix64_uniform(seed, 32)
-    u = np.zeros(len(X))
-    for m in range(8):
-        k = 1 + np.floor(r[4 * m:4 * m + 3] * 4)
-        u += 0.1 * np.sin(2 * np.pi * (X @ k) + 2 * np.pi * r[4 * m + 3])
-    return u
+    from dohp_amd.mesh import sine_modes_state
+    return sine_modes_state(X, seed)
 
 
-def alg_bytes(s, n_tot, n_int, n, nelem, nvert, jv, nq=0, maps=False):
-    """Minimum unique HBM traffic of one apply: x read and y written once (s bytes per value), plus
-    either the stored quadrature data (nq values per point, n^3 points per element) or the Q1 vertex
-    coordinates for geometry recomputed in the kernel and, for the recomputed Jacobian, the state on
-    the closure.  maps: the kernel also reads an element -> node index table (4 bytes per node) and an
-    element -> vertex table (the block-space kernels compute both from the block numbering)."""
-    if nq:
-        return 2 * s * n_int + nq * s * n ** 3 * nelem
-    b = 2 * s * n_int + 24 * nvert + ((4 * n ** 3 + 32) * nelem if maps else 0)
+def alg_bytes(s, n_tot, n_int, n, nelem, nvert, jv):
+    """Algorithmic bytes of one operator application, geometry recomputed from the Q1 vertices (s bytes per
+    value, FP64 geometry):
+        B_lin(s) = s N_tot (read x) + s N_int (write y) + 4 n^3 E (int32 element -> node map)
+                   + 24 N_vert + 32 E (FP64 vertex coordinates + int32 element -> vertex map)
+        B_Jv(s)  = B_lin(s) + s N_tot (read the linearization state u)."""
+    b = s * n_tot + s * n_int + 4 * n ** 3 * nelem + 24 * nvert + 32 * nelem
     return b + (s * n_tot if jv else 0)
+
+
+def stored_bytes(s, n_int, n, nelem, nq=6):
+    """Bytes the stored-data form of an application streams: x read, y written, nq values per Gauss point."""
+    return 2 * s * n_int + nq * s * n ** 3 * nelem
+
+
+def triad_gbs():
+    """STREAM triad a = b + s c over three 1 GiB FP64 arrays (24 bytes per entry), best of 10, GB/s."""
+    import torch
+    n = 1 << 27
+    b = torch.ones(n, dtype=torch.float64, device="cuda")
+    c = torch.ones(n, dtype=torch.float64, device="cuda")
+    a = torch.empty_like(b)
+    for _ in range(3):
+        torch.add(b, c, alpha=3.0, out=a)
+    torch.cuda.synchronize()
+    best = 1e9
+    for _ in range(10):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        torch.add(b, c, alpha=3.0, out=a)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) / 1e3)
+    del a, b, c
+    torch.cuda.empty_cache()
+    return 24 * n / best / 1e9
 
 
 def time_fn(torch, fn, reps, warm=2):
@@ -102,24 +123,38 @@ def time_fn(torch, fn, reps, warm=2):
     return t0.elapsed_time(t1) / 1e3 / reps
 
 
-def cpu_baseline(nthreads):
-    """Oracle J(u) v on a bounded sample of the same workload: 20^3 elements, p=6."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(nthreads, reps=10):
+    """Oracle J(u) v (the reference's per-element arithmetic in C, OpenMP over elements) on a bounded sample of
+    the same workload, 20^3 elements at p = 6: median of `reps` applies after 2 warm-ups, on nthreads threads and
+    on one."""
     from dohp_amd.mesh import block_vertices, splitmix64_uniform
     from oracle import oracle as O
     M = (20, 20, 20)
     V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1)
     S = O.BlockSpace(M, 6, V)
-    E = O.Ellip(S, p=1.5, eps=np.sqrt(0.005), nthreads=nthreads)
     uc = sine_state(S.nodal_coordinates(), 3)
     v = np.concatenate([2 * splitmix64_uniform(9, S.nowned) - 1, np.zeros(S.nclosure - S.nowned)])
-    E.jacobian_apply(uc, v)
-    reps, t = 0, 0.0
-    while t < 10.0 and reps < 20:
-        t0 = time.perf_counter()
-        E.jacobian_apply(uc, v)
-        t += time.perf_counter() - t0
-        reps += 1
-    return S.nowned / (t / reps) / 1e9, f"20^3 elements p=6 J(u)v, mean of {reps} applies"
+    out = {}
+    for nt in (nthreads, 1):
+        E = O.Ellip(S, p=1.5, eps=np.sqrt(0.005), nthreads=nt)
+        ts = []
+        for r in range(reps + 2):
+            t0 = time.perf_counter()
+            E.jacobian_apply(uc, v)
+            if r >= 2:
+                ts.append(time.perf_counter() - t0)
+        out[nt] = S.nowned / float(np.median(ts)) / 1e9
+    return out[nthreads], out[1], f"20^3 elements p=6 J(u)v (N_int {S.nowned}), median of {reps} applies after 2 warm-ups"
 
 
 def measured_traffic(workload):
@@ -151,6 +186,57 @@ def copy_probe():
     t_read = time_fn(torch, lambda: big.sum(), 10)
     print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 3,
                       "copy_GBs": round(16 * n / t_copy / 1e9, 1), "read_GBs": round(32 * n / t_read / 1e9, 1)}))
+
+
+def lognormal_problem(C, torch, Me, p):
+    """Log-normal Poisson problem of the mixed-precision solve: (fs, op, b, x_exact)."""
+    from dohp_amd.fields import lognormal_field
+    from dohp_amd.mesh import block_vertices
+    M = (Me, Me, Me)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1)
+    fs = C.FS(M, p, V)
+    op = C.Ellip(fs, p=2.0, kvert=lognormal_field(V, seed=11, corr_len=0.1))
+    X = fs.nodal_coordinates()[:fs.nowned]
+    xs = torch.as_tensor(sine_state(X, 5) + X[:, 0] * (1 - X[:, 0]), dtype=torch.float64, device="cuda")
+    b = torch.zeros_like(xs)
+    op.mat_mult(xs, b)
+    return fs, op, b, xs
+
+
+def timed_solve(torch, op, q1, b, xs, **kw):
+    """One warm-up solve (kernels, allocations, multigrid hierarchy), then one timed solve from zero: iterations,
+    seconds, the FP64 relative residual computed apart and the error against the manufactured solution."""
+    x = torch.zeros_like(b)
+    op.solve(q1, b, x, rtol=1e-10, **kw)
+    x.zero_()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    st = op.solve(q1, b, x, rtol=1e-10, **kw)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t
+    r = torch.zeros_like(x)
+    op.mat_mult(x, r)
+    return {"seconds": round(t, 4), "outer_its": st["outer_its"], "inner_its": st["inner_its"],
+            "fallback": st["fallback"], "relres": float(torch.linalg.norm(b - r) / torch.linalg.norm(b)),
+            "error_vs_exact": float(torch.linalg.norm(x - xs) / torch.linalg.norm(xs))}
+
+
+def solve_summary(C, torch, Me=48, p=5):
+    """The mixed-precision solve (log-normal k, 48^3, p = 5) for the headline line: plain FP64 against the
+    FP32-inner/FP64-outer defect correction and FP64 CG with the FP32 preconditioner, reference-arithmetic
+    embedded-Q1 matrix, multigrid V-cycle."""
+    fs, op, b, xs = lognormal_problem(C, torch, Me, p)
+    q1 = op.assemble_q1(torch.zeros_like(xs))
+    runs = {name: timed_solve(torch, op, q1, b, xs, mixed=m) for name, m in
+            (("plain_f64", 0), ("mixed_f32_f64", 1), ("f64_cg_f32_precond", 2))}
+    out = {"workload": f"log-normal k Poisson, {Me}^3 hex, p={p}, CG + multigrid on the embedded Q1, to 1e-10",
+           "n_int": fs.nowned, "runs": runs,
+           "speedup_mixed_vs_plain": round(runs["plain_f64"]["seconds"] / runs["mixed_f32_f64"]["seconds"], 3),
+           "speedup_f32_precond_vs_plain": round(runs["plain_f64"]["seconds"] /
+                                                 runs["f64_cg_f32_precond"]["seconds"], 3)}
+    del fs, op, q1, b, xs
+    torch.cuda.empty_cache()
+    return out
 
 
 def solve_workload(args):
@@ -276,17 +362,8 @@ def learned_workload(args):
 
 
 def batch_problem(Me=32, nprob=256):
-    """(mixed-mesh degrees, element vertices, [(M, degree, V)] of the independent problems)."""
-    from dohp_amd.mesh import block_vertices, element_vertices, uniform_integers
-    M = (Me, Me, Me)
-    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=3)
-    deg = uniform_integers(4, Me ** 3, 2, 10)
-    ns, ps = uniform_integers(6, nprob, 4, 16), uniform_integers(7, nprob, 2, 8)
-    probs = []
-    for i in range(nprob):
-        Mi = (int(ns[i]),) * 3
-        probs.append((Mi, int(ps[i]), block_vertices(Mi, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1000 + i)))
-    return deg, element_vertices(M, V), probs
+    from dohp_amd.mesh import mixed_order_batch
+    return mixed_order_batch(Me, nprob)
 
 
 def batch_workload(args):
@@ -303,7 +380,7 @@ def batch_workload(args):
         fs = C.FS(Mi, pi, Vi)
         segs.append(B.add_space(fs))
         for s in (4, 8):
-            nbytes[s] += alg_bytes(s, fs.nclosure, fs.nowned, pi + 1, fs.nelem, fs.nvert, False, maps=True)
+            nbytes[s] += alg_bytes(s, fs.nclosure, fs.nowned, pi + 1, fs.nelem, fs.nvert, False)
         del fs
     nexp = int(((deg + 1) ** 3).sum())
     for s in (4, 8):  # element-local: read x, write y, 8 vertex coordinates per element
@@ -455,6 +532,7 @@ def main():
     ap.add_argument("--elems", type=int, default=None)
     ap.add_argument("--degree", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="leave the mixed-precision solve out of the apply line")
     ap.add_argument("--scatter", type=int, default=1, help="0 = eight-colour deterministic, 1 = atomics")
     ap.add_argument("--workload", choices=["apply", "solve", "learned", "batch", "newton"], default="apply")
     ap.add_argument("--copy-probe", action="store_true")
@@ -532,9 +610,12 @@ def main():
 
     def timed(name, fn, s, jv, nq):
         t = time_fn(torch, fn, reps)
-        b = alg_bytes(s, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, jv, nq=nq)
+        b = alg_bytes(s, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, jv)
         variants[name] = {"us": round(t * 1e6, 1), "gdofs": round(fs.nowned / t / 1e9, 2),
                           "hbm_frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}
+        if nq:  # what the stored-data form streams, as a fraction of the same peak
+            variants[name]["stored_stream_frac"] = round(stored_bytes(s, fs.nowned, n, fs.nelem, nq) / t / 1e9 /
+                                                         HBM_PEAK_GBS, 4)
         print(f"# {name} {t * 1e6:.1f} us", file=sys.stderr, flush=True)
         return t
 
@@ -579,10 +660,25 @@ def main():
     variants["tuning"] = tuning
     op.set_storage(2)
 
-    B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True, nq=6 if stored64 == 1 else 0)
-    achieved = B / (t_kernel or t_step) / 1e9
+    B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True)
+    t_k = t_kernel or t_step
+    achieved = B / t_k / 1e9
     form = "from stored quadrature data, " if stored64 == 1 else ""
     workload = f"nonlinear diffusion J(u0)v {form}{Me}^3 hex, p={p}, FP64, perturbed mesh"
+    triad = triad_gbs()
+    traffic = measured_traffic(workload)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel_us": round(t_k * 1e6, 1),
+                "algorithmic_bytes": int(B),
+                "byte_model": "B_Jv(8) = 8 N_tot + 8 N_int + 4 n^3 E + 24 N_vert + 32 E + 8 N_tot (geometry "
+                              "recomputed from the Q1 vertices)",
+                "measured_triad_GBs": round(triad, 1), "frac_vs_measured_triad": round(achieved / triad, 4)}
+    if traffic:
+        roofline["traffic_over_algorithmic"] = round(traffic / B, 3)
+    if stored64 == 1:  # the stored form streams its per-point tensor: what it reaches on its own bytes
+        sb = stored_bytes(8, fs.nowned, n, fs.nelem)
+        roofline["stored_data_stream"] = {"bytes": int(sb), "GBs": round(sb / t_k / 1e9, 1),
+                                          "frac": round(sb / t_k / 1e9 / HBM_PEAK_GBS, 4)}
     cpu = None
     if not args.no_cpu:
         ncores = os.cpu_count() or 1
@@ -590,8 +686,9 @@ def main():
             ncores = len(os.sched_getaffinity(0))
         except AttributeError:
             pass
-        val, sample = cpu_baseline(ncores)
-        cpu = {"value": round(val, 4), "unit": "GDOF/s", "cores": ncores, "kind": "port", "sample": sample}
+        val, val1, sample = cpu_baseline(ncores)
+        cpu = {"value": round(val, 4), "unit": "GDOF/s", "cores": ncores, "kind": "port", "sample": sample,
+               "value_1core": round(val1, 5), "cpu_model": cpu_model(), "nproc": os.cpu_count()}
     out = {
         "metric": "GDOF/s operator apply at order 6",
         "value": round(fs.nowned / t_step / 1e9, 3),
@@ -608,12 +705,15 @@ def main():
         "config": {"workload": workload, "elements": fs.nelem, "degree": p, "n_int": fs.nowned,
                    "n_tot": fs.nclosure, "scatter": "colored" if args.scatter == 0 else "atomic",
                    "jacobian_data": "stored per Gauss point (6 FP64 values)" if stored64 == 1 else "recomputed"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload),
-                     "kernel_us": round((t_kernel or t_step) * 1e6, 1), "algorithmic_bytes": int(B)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "variants": variants,
     }
+    if not args.no_solve:
+        del op, lin, u, v, v32, f, y, y32, fs
+        torch.cuda.empty_cache()
+        out["solve"] = solve_summary(C, torch)
+        print(f"# solve: {json.dumps(out['solve'])}", file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 

@@ -22,7 +22,7 @@ INSERT_VALUES, ADD_VALUES = 1, 2
 HOMOGENEOUS, INHOMOGENEOUS = 0, 1
 QUAD_FAST, QUAD_SPARSE, QUAD_SELF = 0, 1, 2
 GAUSS_LEGENDRE, GAUSS_LOBATTO = 0, 1
-ERR_NEGATIVE_JACOBIAN = 77
+ERR_ARG, ERR_NEGATIVE_JACOBIAN, ERR_NAN, ERR_OOM, ERR_HIP = 62, 77, 78, 98, 99
 
 if not os.path.exists(LIBPATH):
     raise ImportError(f"dohp_amd: HIP library not built ({LIBPATH}); run __graft_entry__.build()")
@@ -50,6 +50,7 @@ _proto("dohpTensorRule", ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp
 _proto("dohpTensorBasis", ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp)
 _proto("dEFSApplyHexBatched", ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp,
        ctypes.c_int, ctypes.c_int, _vp)
+_proto("dRuleComputePhysicalBatched", ctypes.c_int, _dp, ctypes.c_int, _vp, _vp, _vp, _vp)
 _proto("dFSCreateBlock", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(_vp))
 _proto("dFSDestroy", ctypes.POINTER(_vp))
 _proto("dFSGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long), ctypes.POINTER(_long), _ip)
@@ -66,6 +67,7 @@ _proto("dohpEllipShellMatMult", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMultF32", _vp, _vp, _vp, _vp)
 _proto("dohpEllipShellMatMultAdd", _vp, _vp, _vp, _vp)
 _proto("dohpEllipSetScatter", _vp, ctypes.c_int)
+_proto("dohpEllipSetChecks", _vp, ctypes.c_int)
 _proto("dohpEllipGetTuning", _vp, ctypes.c_int, ctypes.c_int, _vp, _vp)
 _proto("dohpEllipSetShape", _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int)
 _proto("dohpEllipAssembleQ1", _vp, _vp, ctypes.POINTER(_vp), _vp)
@@ -283,6 +285,14 @@ def efs_apply_hex_batched(P, rcoord, nelem, jinv, D, x, y, amode, imode=INSERT_V
                                    _stream(stream)))
 
 
+def rule_compute_physical_batched(weight1d, nelem, jac, jinv, jw, stream=None):
+    """dRuleComputePhysical over nelem elements of a tensor rule with 1-D weights weight1d: jac, jinv
+    [nelem][Q^3][9] and jw [nelem][Q^3] are device tensors (float64).  Raises DohpError with code
+    ERR_NEGATIVE_JACOBIAN or ERR_NAN when the device finds a negative or non-finite determinant."""
+    w, wp = _np(weight1d)
+    check(_lib.dRuleComputePhysicalBatched(w.size, wp, int(nelem), _ptr(jac), _ptr(jinv), _ptr(jw), _stream(stream)))
+
+
 # ---------------------------------------------------------------------- dFS
 class FS:
     """Continuous degree-p space on a structured hexahedral block (dFS)."""
@@ -419,6 +429,11 @@ class Ellip:
 
     def set_scatter(self, mode):
         check(_lib.dohpEllipSetScatter(self.h, int(mode)))
+
+    def set_checks(self, mode):
+        """1 (default): function and mat_mult* wait for their stream and raise DohpError(ERR_NAN) when a
+        non-finite value met the element kernels; 0: asynchronous, unchecked."""
+        check(_lib.dohpEllipSetChecks(self.h, int(mode)))
 
     def set_q1_mode(self, mode):
         """Later embedded-Q1 assemblies: 0 = the reference's arithmetic, 1 = corrected (GLL sub-mesh Q1 FEM)."""

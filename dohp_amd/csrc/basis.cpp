@@ -181,8 +181,8 @@ extern "C" dErr dohpTensorBasis(int P, int Q, const double *rcoord, double *inte
   DOHP_CHECK(Q >= 1 && Q <= 4 * kMaxN, dohpERR_ARG, "rule size Q must be in [1,64]");
   double z[kMaxN], w[kMaxN], Dc[kMaxN * kMaxN];
   gauss_lobatto(P, z, w);
-  std::vector<double> B(Q * P);
-  lagrange_matrix(P, z, Q, rcoord, B.data());
+  double B[4 * kMaxN * kMaxN];  // Q x P, on the stack (no allocation: used by the allocation-free apply)
+  lagrange_matrix(P, z, Q, rcoord, B);
   collocation_derivative(P, z, Dc);
   for (int q = 0; q < Q; q++)
     for (int j = 0; j < P; j++) {

@@ -21,6 +21,7 @@
 // precision.  From degree 4 up, FP32 applies may also offer the tuner a third tile whose contractions are
 // wavefront-level GEMMs on the matrix cores (BF16 operands, FP32 accumulation; dohpBatchSetMFMA).
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -414,6 +415,7 @@ struct _p_dohpBatch {
   DevBuf<int4> items[2];
   int nitems[2] = {0, 0};
   size_t lds[2] = {0, 0};
+  std::map<std::pair<int, size_t>, long> grid[2];  // persistent grid per (device, dynamic LDS bytes)
   _p_dohpBatch()
   {
     for (int p = 0; p < 2; p++)
@@ -582,14 +584,19 @@ dErr batch_launch(_p_dohpBatch *b, const int4 *items, int nitems, size_t lds, co
   hipLaunchKernelGGL(batch_prep_kernel<T>, dim3(b->nzero > 0 ? std::min<long>((b->nzero + 255) / 256, 2048) : 1),
                      dim3(256), 0, st, y, b->nzero, b->counter.ptr);
   if (nitems == 0) return 0;
-  int dev = 0, per_cu = 0;
-  hipDeviceProp_t prop;
-  DOHP_HIP(hipGetDevice(&dev));
-  DOHP_HIP(hipGetDeviceProperties(&prop, dev));
-  DOHP_CHECK(lds <= prop.sharedMemPerBlock, dohpERR_ARG, "batch tile exceeds the workgroup LDS limit");
+  int dev = 0, cus = 0;
+  DOHP_CALL(current_device_cus(dev, cus));
+  DOHP_CHECK(lds <= 160 * 1024, dohpERR_ARG, "batch tile exceeds the workgroup LDS limit");
+  // (the kernel's dynamic-LDS limit is shared by every batch object: set for this launch's size)
   DOHP_HIP(hipFuncSetAttribute((const void *)batch_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  DOHP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<T>, kBT, lds));
-  const long grid = std::min<long>(nitems, (long)prop.multiProcessorCount * std::max(per_cu, 1));
+  // grid per (device, dynamic LDS), asked once: the launch shape does not change between applications
+  long &grid_cached = b->grid[sizeof(T) == 4][{dev, lds}];
+  if (!grid_cached) {
+    int per_cu = 0;
+    DOHP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<T>, kBT, lds));
+    grid_cached = (long)cus * std::max(per_cu, 1);
+  }
+  const long grid = std::min<long>(nitems, grid_cached);
   BArgs a = b->args;
   a.items = items;
   a.nitems = nitems;

@@ -72,17 +72,21 @@ struct DevBuf {
     }
     return *this;
   }
+  // On failure the buffer is left empty (ptr = nullptr, n = 0), so a later size check reallocates.
   dErr alloc(size_t count) {
     release();
-    n = count;
     if (count == 0) return 0;
     if (hipMalloc(&ptr, count * sizeof(T)) != hipSuccess) {
       ptr = nullptr;
+      (void)hipGetLastError();
       set_error("hipMalloc failed for " + std::to_string(count * sizeof(T)) + " bytes");
-      return dohpERR_HIP;
+      return dohpERR_OOM;
     }
+    n = count;
     return 0;
   }
+  // Holds exactly `count` elements (reallocated otherwise).
+  bool holds(size_t count) const { return ptr != nullptr && n == count; }
   dErr upload(const T *h, size_t count) {
     DOHP_CALL(alloc(count));
     if (count) DOHP_HIP(hipMemcpy(ptr, h, count * sizeof(T), hipMemcpyHostToDevice));
@@ -96,6 +100,21 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// Current device and its compute-unit count (the properties are read once per device).
+inline dErr current_device_cus(int &dev, int &cus)
+{
+  static int cached[64] = {};
+  DOHP_HIP(hipGetDevice(&dev));
+  DOHP_CHECK(dev >= 0 && dev < 64, dohpERR_ARG, "device index out of range");
+  if (!cached[dev]) {
+    hipDeviceProp_t prop;
+    DOHP_HIP(hipGetDeviceProperties(&prop, dev));
+    cached[dev] = prop.multiProcessorCount;
+  }
+  cus = cached[dev];
+  return 0;
+}
+
 // Logical block index: consecutive workgroups are dealt round-robin to the eight XCDs, so
 // give each XCD a contiguous range of work (neighbours share data through its L2).
 __device__ inline long xcd_block(long b, long nb)
@@ -108,6 +127,9 @@ __device__ inline long xcd_block(long b, long nb)
 
 // Internal entry points shared between the operator and the Q1 assembly.
 extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hipStream_t st);
+// y = J x without the entry points' non-finite check (no wait for the stream): the library's solvers.
+dErr dohpEllipMatMult_Private(dohpEllip op, const double *x, double *y, hipStream_t st);
+dErr dohpEllipMatMult_Private(dohpEllip op, const float *x, float *y, hipStream_t st);
 
 // Sparse matrix on the owned nodes.  CSR is the assembled form; on a block space the owned nodes
 // form a box of the node lattice (nx * ny * nz, lexicographic, last index fastest) and every row

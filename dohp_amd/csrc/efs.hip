@@ -116,28 +116,113 @@ __global__ void efs_apply_kernel(int P, int Q, const double *__restrict__ tab, c
   }
 }
 
+// 1-D weights of a tensor rule, passed by value.
+struct RuleW {
+  double w[kMaxN];
+};
+
+// dRuleComputePhysical for every point of nelem elements (one thread per point): the cofactor inverse of
+// dGeomInvert3 (include/dohpgeom.h:57-75, same operation order), jw = w_i w_j w_k det.  flag bit 0: a
+// negative determinant (jacobi.c:397); bit 1: a non-finite one.
+__global__ void rule_physical_kernel(int Q, RuleW rw, long npts, const double *__restrict__ jac,
+                                     double *__restrict__ jinv, double *__restrict__ jw, int *__restrict__ flag)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  const int Q3 = Q * Q * Q, q = (int)(t % Q3), i = q / (Q * Q), j = (q / Q) % Q, k = q % Q;
+  double a[9];
+#pragma unroll
+  for (int r = 0; r < 9; r++) a[r] = jac[t * 9 + r];
+  const double b0 = (a[1 * 3 + 1] * a[2 * 3 + 2] - a[2 * 3 + 1] * a[1 * 3 + 2]);
+  const double b3 = -(a[1 * 3 + 0] * a[2 * 3 + 2] - a[2 * 3 + 0] * a[1 * 3 + 2]);
+  const double b6 = (a[1 * 3 + 0] * a[2 * 3 + 1] - a[2 * 3 + 0] * a[1 * 3 + 1]);
+  const double det = a[0] * b0 + a[1] * b3 + a[2] * b6;
+  const double id = 1.0 / det;
+  double *b = jinv + t * 9;
+  b[0] = id * b0;
+  b[1] = -id * (a[0 * 3 + 1] * a[2 * 3 + 2] - a[2 * 3 + 1] * a[0 * 3 + 2]);
+  b[2] = id * (a[0 * 3 + 1] * a[1 * 3 + 2] - a[1 * 3 + 1] * a[0 * 3 + 2]);
+  b[3] = id * b3;
+  b[4] = id * (a[0 * 3 + 0] * a[2 * 3 + 2] - a[2 * 3 + 0] * a[0 * 3 + 2]);
+  b[5] = -id * (a[0 * 3 + 0] * a[1 * 3 + 2] - a[1 * 3 + 0] * a[0 * 3 + 2]);
+  b[6] = id * b6;
+  b[7] = -id * (a[0 * 3 + 0] * a[2 * 3 + 1] - a[2 * 3 + 0] * a[0 * 3 + 1]);
+  b[8] = id * (a[0 * 3 + 0] * a[1 * 3 + 1] - a[1 * 3 + 0] * a[0 * 3 + 1]);
+  jw[t] = rw.w[i] * rw.w[j] * rw.w[k] * det;
+  const int f = (det < 0.0 ? 1 : 0) | (__builtin_isfinite(det) ? 0 : 2);
+  if (f) atomicOr(flag, f);
+}
+
 }  // namespace
 
-extern "C" dErr dEFSApplyHexBatched(int P, int Q, const double *rcoord, int nelem, const double *jinv, int D,
-                                    const double *in, double *out, int amode, int imode, void *stream)
+extern "C" dErr dRuleComputePhysicalBatched(int Q, const double *weight, int nelem, const double *jac, double *jinv,
+                                            double *jw, void *stream)
 {
-  DOHP_CHECK(P >= 2 && P <= 12 && Q >= 1 && Q <= 12, dohpERR_ARG, "dEFSApplyHexBatched supports 2<=P<=12, 1<=Q<=12");
+  DOHP_CHECK(Q >= 1 && Q <= kMaxN, dohpERR_ARG, "dRuleComputePhysicalBatched supports 1 <= Q <= 16");
+  DOHP_CHECK(nelem >= 0, dohpERR_ARG, "negative element count");
+  DOHP_CHECK(weight && jac && jinv && jw, dohpERR_ARG, "null argument");  // jacobi.c:389-390
+  if (nelem == 0) return 0;
+  RuleW rw{};
+  for (int q = 0; q < Q; q++) rw.w[q] = weight[q];
+  hipStream_t st = (hipStream_t)stream;
+  DevBuf<int> flag;
+  DOHP_CALL(flag.alloc(1));
+  DOHP_HIP(hipMemsetAsync(flag.ptr, 0, sizeof(int), st));
+  const long npts = (long)nelem * Q * Q * Q;
+  hipLaunchKernelGGL(rule_physical_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, st, Q, rw, npts, jac,
+                     jinv, jw, flag.ptr);
+  DOHP_HIP(hipGetLastError());
+  int h = 0;
+  DOHP_HIP(hipMemcpyAsync(&h, flag.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+  DOHP_HIP(hipStreamSynchronize(st));
+  DOHP_CHECK(!(h & 1), dohpERR_NEGATIVE_JACOBIAN, "Negative Jacobian at a quadrature point");
+  DOHP_CHECK(!(h & 2), dohpERR_NAN, "Non-finite Jacobian determinant at a quadrature point");
+  return 0;
+}
+
+namespace {
+
+constexpr int kEfsMaxP = 12;
+constexpr int kEfsTab = 2 * kEfsMaxP * kEfsMaxP;  // B and D, [Q][P] each
+
+dErr efs_check(int P, int Q, int nelem, const double *rcoord, const double *jinv, int D, const double *in,
+               double *out, int amode, int imode)
+{
+  DOHP_CHECK(P >= 2 && P <= kEfsMaxP && Q >= 1 && Q <= kEfsMaxP, dohpERR_ARG,
+             "dEFSApplyHexBatched supports 2<=P<=12, 1<=Q<=12");
+  DOHP_CHECK(nelem >= 0, dohpERR_ARG, "negative element count");
   DOHP_CHECK(D >= 1, dohpERR_ARG, "dofs must be positive");  // jacobi.c:495
   DOHP_CHECK(imode == dINSERT_VALUES || imode == dADD_VALUES, dohpERR_ARG, "InsertMode not supported");
   DOHP_CHECK(amode >= dAPPLY_INTERP && amode <= dAPPLY_GRAD_TRANSPOSE, dohpERR_ARG, "invalid dApplyMode");
   DOHP_CHECK(amode != dAPPLY_GRAD_TRANSPOSE || jinv, dohpERR_ARG, "No Jinv, need a mapping");
   DOHP_CHECK(rcoord && in && out, dohpERR_ARG, "null argument");
+  return 0;
+}
+
+dErr efs_launch(int P, int Q, const double *dtab, int nelem, const double *jinv, int D, const double *in, double *out,
+                int amode, int imode, hipStream_t st)
+{
+  const int M = std::max(P, Q), M3 = M * M * M;
+  const size_t shmem = sizeof(double) * (2 * Q * P + 3 * M3);
+  hipLaunchKernelGGL(efs_apply_kernel, dim3(nelem), dim3(256), shmem, st, P, Q, dtab, jinv, D, in, out, amode,
+                     imode);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" dErr dEFSApplyHexBatched(int P, int Q, const double *rcoord, int nelem, const double *jinv, int D,
+                                    const double *in, double *out, int amode, int imode, void *stream)
+{
+  DOHP_CALL(efs_check(P, Q, nelem, rcoord, jinv, D, in, out, amode, imode));
   if (nelem == 0) return 0;
   std::vector<double> tab(2 * Q * P);
   DOHP_CALL(dohpTensorBasis(P, Q, rcoord, tab.data(), tab.data() + Q * P, nullptr, nullptr));
   DevBuf<double> dtab;
   DOHP_CALL(dtab.upload(tab.data(), tab.size()));
-  const int M = std::max(P, Q), M3 = M * M * M;
-  const size_t shmem = sizeof(double) * (2 * Q * P + 3 * M3);
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(efs_apply_kernel, dim3(nelem), dim3(256), shmem, st, P, Q, dtab.ptr, jinv, D, in, out, amode,
-                     imode);
-  DOHP_HIP(hipGetLastError());
+  DOHP_CALL(efs_launch(P, Q, dtab.ptr, nelem, jinv, D, in, out, amode, imode, st));
   DOHP_HIP(hipStreamSynchronize(st));  // dtab is freed on return
   return 0;
 }

@@ -74,7 +74,8 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
                                                     const T *__restrict__ ustate, const T *__restrict__ xdir,
                                                     T *__restrict__ y, long nowned, int atomic,
-                                                    const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph)
+                                                    const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph,
+                                                    int *__restrict__ nanflag)
 {
   constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
   __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
@@ -175,15 +176,17 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
   __syncthreads();
   int tid_body = tid;  // derived values are formed here, not hoisted above the gather
   asm volatile("" : "+v"(tid_body));
+  bool bad = false;  // a non-finite contribution was scattered (NaN or Inf in the inputs or the state)
 #define DOHP_ELEM(l) \
   (lb * EPB + (l) < ne ? (elist ? (long)elist[lb * EPB + (l)] : lb * EPB + (l)) : -1L)
 #define DOHP_BATCH_HOOK() ((void)0)
-#define DOHP_SCATTER(c, val) \
-  do {                       \
-    if (atomic)              \
-      atomicAdd(&y[c], val); \
-    else                     \
-      y[c] += val;           \
+#define DOHP_SCATTER(c, val)            \
+  do {                                  \
+    bad |= !__builtin_isfinite(val);    \
+    if (atomic)                         \
+      atomicAdd(&y[c], val);            \
+    else                                \
+      y[c] += val;                      \
   } while (0)
 #define DOHP_QD_PRELOADED
 #include "ellip_batch.inc"
@@ -191,6 +194,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
 #undef DOHP_SCATTER
 #undef DOHP_BATCH_HOOK
 #undef DOHP_ELEM
+  if (bad) atomicOr(nanflag, 1);
 }
 
 // Persistent form of the kernel (one wavefront per workgroup, atomic scatter): each workgroup
@@ -202,7 +206,7 @@ __global__ void __launch_bounds__(wg_threads(N, EPB))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1))) ellip_persistent_kernel(
     long nbatch, long ne, BoxMap bm, const double *__restrict__ vxyz, const double *__restrict__ kv,
     const T *__restrict__ ustate, const T *__restrict__ xdir, T *__restrict__ y, long nowned,
-    const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph)
+    const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph, int *__restrict__ nanflag)
 {
   constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
   // the Jacobian action prefetches its two fields (state, direction) only in FP32 (in FP64 the
@@ -274,6 +278,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
     load_idx(bt);
     load_val(bt);
   }
+  bool bad = false;  // a non-finite contribution was scattered
   for (; bt < hi; bt += per) {
     if constexpr (PREFETCH_VALUES) {
 #pragma unroll
@@ -322,12 +327,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
   do {                    \
     if (nxt < hi) load_val(nxt); \
   } while (0)
-#define DOHP_SCATTER(c, val) \
-  do {                       \
-    if (atomic)              \
-      atomicAdd(&y[c], val); \
-    else                     \
-      y[c] += val;           \
+#define DOHP_SCATTER(c, val)            \
+  do {                                  \
+    bad |= !__builtin_isfinite(val);    \
+    if (atomic)                         \
+      atomicAdd(&y[c], val);            \
+    else                                \
+      y[c] += val;                      \
   } while (0)
 #include "ellip_batch.inc"
 #undef DOHP_SCATTER
@@ -335,6 +341,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
 #undef DOHP_ELEM
     __syncthreads();  // the next batch overwrites the tiles
   }
+  if (bad) atomicOr(nanflag, 1);
 }
 
 template <typename T>
@@ -381,8 +388,17 @@ struct _p_dohpEllip {
   DevBuf<double> forcing;                     // assembled forcing term -(f, phi) on the owned nodes
   bool has_forcing = false;
   DevBuf<float> state32, tab32;
+  // set by the element kernels when a scattered contribution is not finite (a NaN or Inf in the input
+  // vector or the state); the checked entry points clear it, wait for their stream and read it back
+  DevBuf<int> nanflag;
+  int *nan_host = nullptr;  // pinned
+  int checks = 1;
   // host tables
   std::vector<double> B, Dh, w, phi;
+  ~_p_dohpEllip()
+  {
+    if (nan_host) (void)hipHostFree(nan_host);
+  }
 };
 
 namespace {
@@ -440,14 +456,14 @@ void launch_shape(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStr
     const long ne = fs->nelem;
     hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB>), dim3((ne + EPB - 1) / EPB), block, 0, st, nullptr, ne,
                        box_map(fs), fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 1, gtab<T>(op),
-                       op->wphi.ptr, ph);
+                       op->wphi.ptr, ph, op->nanflag.ptr);
   } else {
     for (int col = 0; col < 8; col++) {
       const long ne = fs->color_off[col + 1] - fs->color_off[col];
       if (!ne) continue;
       hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB>), dim3((ne + EPB - 1) / EPB), block, 0, st,
                          fs->d_color_elems.ptr + fs->color_off[col], ne, box_map(fs), fs->d_vcoords.ptr,
-                         op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), op->wphi.ptr, ph);
+                         op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), op->wphi.ptr, ph, op->nanflag.ptr);
     }
   }
 }
@@ -459,18 +475,22 @@ dErr launch_persistent(const _p_dohpEllip *op, const T *ust, const T *x, T *y, h
 {
   const dohpFS fs = op->fs;
   const int threads = wg_threads(N, EPB);
-  int dev, per_cu = 0;
-  hipDeviceProp_t prop;
-  DOHP_HIP(hipGetDevice(&dev));
-  DOHP_HIP(hipGetDeviceProperties(&prop, dev));
-  DOHP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ellip_persistent_kernel<N, T, MODE, EPB>, threads, 0));
+  int dev, cus;
+  DOHP_CALL(current_device_cus(dev, cus));
+  static int occupancy[64] = {};  // workgroups per CU of this instance, per device (asked once)
+  if (!occupancy[dev]) {
+    int per_cu = 0;
+    DOHP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ellip_persistent_kernel<N, T, MODE, EPB>, threads,
+                                                          0));
+    occupancy[dev] = per_cu > 0 ? per_cu : 1;
+  }
   const long nbatch = (fs->nelem + EPB - 1) / EPB;
-  long grid = (long)prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
+  long grid = (long)cus * occupancy[dev];
   grid = std::min(grid, ((nbatch + 7) / 8) * 8);
   grid = std::max(8L, grid - grid % 8);
   hipLaunchKernelGGL((ellip_persistent_kernel<N, T, MODE, EPB>), dim3((unsigned)grid), dim3(threads), 0, st, nbatch,
                      (long)fs->nelem, box_map(fs), fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned,
-                     gtab<T>(op), op->wphi.ptr, ph);
+                     gtab<T>(op), op->wphi.ptr, ph, op->nanflag.ptr);
   return 0;
 }
 
@@ -590,7 +610,7 @@ dErr ensure_qd(_p_dohpEllip *op, hipStream_t st)
   // one element's worth of slack at the end: the kernels may load the Bratu component of the last
   // element's data ahead of the (uniform) test that it exists
   const size_t n = ((size_t)fs->nelem * (op->prm.lambda != 0 ? 7 : 6) + 1) * fs->n * fs->n * fs->n;
-  if (op->qd.n != n) DOHP_CALL(op->qd.alloc(n));
+  if (!op->qd.holds(n)) DOHP_CALL(op->qd.alloc(n));
   DOHP_CALL(launch_fixed<MODE_QD>(op, is_linear(op) ? nullptr : op->state.ptr, op->qd.ptr, st));
   op->qd_ok = true;
   op->qd32_ok = false;
@@ -608,7 +628,7 @@ dErr ensure_qd32(_p_dohpEllip *op, hipStream_t st)
   if (op->qd32_ok && op->qd_ok) return 0;
   DOHP_CALL(ensure_qd(op, st));
   const long n = (long)op->qd.n;
-  if (op->qd32.n != op->qd.n) DOHP_CALL(op->qd32.alloc(op->qd.n));
+  if (!op->qd32.holds(op->qd.n)) DOHP_CALL(op->qd32.alloc(op->qd.n));
   hipLaunchKernelGGL(round_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, op->qd.ptr, op->qd32.ptr,
                      n);
   DOHP_HIP(hipGetLastError());
@@ -691,6 +711,13 @@ extern "C" dErr dohpEllipCreate(dohpFS fs, const dohpEllipParam *prm, dohpEllip 
   }
   if (!err) err = op->state.alloc(fs->nclosure);
   if (!err) err = op->state32.alloc(fs->nclosure);
+  if (!err) err = op->nanflag.alloc(1);
+  if (!err && hipMemset(op->nanflag.ptr, 0, sizeof(int)) != hipSuccess) err = dohpERR_HIP;
+  if (!err && hipHostMalloc((void **)&op->nan_host, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    op->nan_host = nullptr;
+    set_error("pinned allocation failed");
+    err = dohpERR_OOM;
+  }
   if (err) {
     delete op;
     return err;
@@ -748,10 +775,25 @@ extern "C" dErr dohpEllipSetCoefficient(dohpEllip op, const double *kvert)
   return op->kvert.upload(kvert, op->fs->nvert);
 }
 
-extern "C" dErr dohpEllipFunction(dohpEllip op, const double *u, double *f, void *stream)
+namespace {
+
+// Checked form of an entry point (checks on): clears the non-finite flag, runs f, waits for the stream
+// and reports a non-finite value met by the element kernels.  Checks off: f alone, asynchronous.
+template <typename F>
+dErr checked(_p_dohpEllip *op, hipStream_t st, const char *what, F f)
 {
-  DOHP_CHECK(op && u && f, dohpERR_ARG, "null argument");
-  hipStream_t st = (hipStream_t)stream;
+  if (!op->checks) return f();
+  DOHP_CALL(zero(op->nanflag.ptr, 1L, st));
+  DOHP_CALL(f());
+  DOHP_HIP(hipMemcpyAsync(op->nan_host, op->nanflag.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+  DOHP_HIP(hipStreamSynchronize(st));
+  DOHP_CHECK(*op->nan_host == 0, dohpERR_NAN,
+             std::string(what) + ": non-finite value met on the device (NaN or Inf in the input vector or the state)");
+  return 0;
+}
+
+dErr ellip_function(_p_dohpEllip *op, const double *u, double *f, hipStream_t st)
+{
   const dohpFS fs = op->fs;
   hipLaunchKernelGGL(lattice_state_kernel, dim3((fs->nclosure + 255) / 256), dim3(256), 0, st, u, fs->d_dcache.ptr,
                      fs->d_lat2clo.ptr, fs->nowned, fs->nclosure, op->state.ptr, op->state32.ptr);
@@ -764,6 +806,22 @@ extern "C" dErr dohpEllipFunction(dohpEllip op, const double *u, double *f, void
     DOHP_CALL(zero(f, fs->nowned, st));
   }
   return launch<double, MODE_FUN>(op, op->state.ptr, nullptr, f, st);
+}
+
+}  // namespace
+
+extern "C" dErr dohpEllipFunction(dohpEllip op, const double *u, double *f, void *stream)
+{
+  DOHP_CHECK(op && u && f, dohpERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  return checked(op, st, "dohpEllipFunction", [&]() { return ellip_function(op, u, f, st); });
+}
+
+extern "C" dErr dohpEllipSetChecks(dohpEllip op, int mode)
+{
+  DOHP_CHECK(op && (mode == 0 || mode == 1), dohpERR_ARG, "checks must be 0 (off) or 1 (on)");
+  op->checks = mode;
+  return 0;
 }
 
 extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hipStream_t st)
@@ -861,12 +919,14 @@ dErr apply_add(_p_dohpEllip *op, const T *x, T *y, hipStream_t st)
     DevBuf<T> scratch;
     DOHP_CALL(scratch.alloc(op->fs->nowned > 0 ? op->fs->nowned : 1));
     float t_qd = 0, t_re = 0;
-    // the data may not fit in device memory: then the recomputing form is kept
-    const bool fits = Prec<T>::ensure(op, st) == 0;
+    // the data may not fit in device memory: then the recomputing form is kept (any other failure is
+    // returned)
+    const dErr fit_err = Prec<T>::ensure(op, st);
+    if (fit_err && fit_err != dohpERR_OOM) return fit_err;
+    const bool fits = fit_err == 0;
     if (fits) {
       DOHP_CALL(time_call([&]() { return apply_stored<T>(op, x, scratch.ptr, st); }, st, t_qd));
     } else {
-      (void)hipGetLastError();  // the failed allocation is not an error of the application
       drop_qd(op);
       op->qd.release();
       op->qd32.release();
@@ -886,27 +946,38 @@ dErr apply_add(_p_dohpEllip *op, const T *x, T *y, hipStream_t st)
 
 }  // namespace
 
+// Unchecked applications for the library's own solvers (their Krylov loops test the norms instead).
+dErr dohpEllipMatMult_Private(dohpEllip op, const double *x, double *y, hipStream_t st)
+{
+  DOHP_CALL(zero(y, op->fs->nowned, st));
+  return apply_add(op, x, y, st);
+}
+dErr dohpEllipMatMult_Private(dohpEllip op, const float *x, float *y, hipStream_t st)
+{
+  DOHP_CALL(zero(y, op->fs->nowned, st));
+  return apply_add(op, x, y, st);
+}
+
 extern "C" dErr dohpEllipShellMatMult(dohpEllip op, const double *x, double *y, void *stream)
 {
   DOHP_CHECK(op && x && y, dohpERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
-  DOHP_CALL(zero(y, op->fs->nowned, st));
-  return apply_add(op, x, y, st);
+  return checked(op, st, "dohpEllipShellMatMult", [&]() { return dohpEllipMatMult_Private(op, x, y, st); });
 }
 
 extern "C" dErr dohpEllipShellMatMultAdd(dohpEllip op, const double *x, double *y, void *stream)
 {
   DOHP_CHECK(op && x && y, dohpERR_ARG, "null argument");
   DOHP_CHECK(op->scatter == 1, dohpERR_ARG, "MatMultAdd accumulates with the atomic scatter only");
-  return apply_add(op, x, y, (hipStream_t)stream);
+  hipStream_t st = (hipStream_t)stream;
+  return checked(op, st, "dohpEllipShellMatMultAdd", [&]() { return apply_add(op, x, y, st); });
 }
 
 extern "C" dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x, float *y, void *stream)
 {
   DOHP_CHECK(op && x && y, dohpERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
-  DOHP_CALL(zero(y, op->fs->nowned, st));
-  return apply_add(op, x, y, st);
+  return checked(op, st, "dohpEllipShellMatMultF32", [&]() { return dohpEllipMatMult_Private(op, x, y, st); });
 }
 
 extern "C" dErr dohpEllipGetStorage(dohpEllip op, int fp32, int *stored, float *ms)

@@ -379,7 +379,7 @@ dErr energy_and_grad(dohpEllip op, dohpMG mg, long nn, const double *s, const do
   hipLaunchKernelGGL(mul_kernel, grid_of(nn), dim3(256), 0, st, nn, s, r, t.ptr);      // t = S r
   DOHP_CALL(dohpMGApply(mg, t.ptr, u.ptr, st));                                        // u = M S r
   hipLaunchKernelGGL(mul_kernel, grid_of(nn), dim3(256), 0, st, nn, s, u.ptr, z.ptr);  // z = S u
-  DOHP_CALL(dohpEllipShellMatMult(op, z.ptr, Jd.ptr, st));                             // J z
+  DOHP_CALL(dohpEllipMatMult_Private(op, z.ptr, Jd.ptr, st));                             // J z
   double zr, zJz, eJe;
   DOHP_CALL(host_dot(z.ptr, r, nn, &zr, part, st));
   DOHP_CALL(host_dot(z.ptr, Jd.ptr, nn, &zJz, part, st));
@@ -392,7 +392,7 @@ dErr energy_and_grad(dohpEllip op, dohpMG mg, long nn, const double *s, const do
   }
   const double a = alpha > 0 ? alpha : zr / zJz;
   hipLaunchKernelGGL(axpy_kernel, grid_of(nn), dim3(256), 0, st, nn, -a, z.ptr, e, d.ptr);  // d = e - a z
-  DOHP_CALL(dohpEllipShellMatMult(op, d.ptr, Jd.ptr, st));                                   // J d
+  DOHP_CALL(dohpEllipMatMult_Private(op, d.ptr, Jd.ptr, st));                                   // J d
   double dJd;
   DOHP_CALL(host_dot(d.ptr, Jd.ptr, nn, &dJd, part, st));
   if (energy) *energy = std::max(dJd, 0.0) / eJe;

@@ -332,9 +332,14 @@ dErr dohpMatPrepareStencil_Private(_p_dohpMat *A, dohpFS fs, bool *box, hipStrea
   A->box[0] = b.nx;
   A->box[1] = b.ny;
   A->box[2] = b.nz;
-  if (!A->st64.ptr) {
-    DOHP_CALL(A->st64.alloc(27 * b.n()));
-    DOHP_CALL(A->st32.alloc(27 * b.n()));
+  if (!A->st64.holds(27 * b.n()) || !A->st32.holds(27 * b.n())) {  // both or neither
+    dErr err = A->st64.alloc(27 * b.n());
+    if (!err) err = A->st32.alloc(27 * b.n());
+    if (err) {
+      A->st64.release();
+      A->st32.release();
+      return err;
+    }
   }
   *box = true;
   return 0;

@@ -490,7 +490,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
     tab.insert(tab.end(), tb.hsub, tb.hsub + pp);
     for (int q = 0; q < nq2; q++) tab.insert(tab.end(), tb.Bs[q], tb.Bs[q] + nn);
     for (int q = 0; q < nq2; q++) tab.insert(tab.end(), tb.Ds[q], tb.Ds[q] + nn);
-    if (A->q1tab.n != tab.size()) DOHP_CALL(A->q1tab.alloc(tab.size()));
+    if (!A->q1tab.holds(tab.size())) DOHP_CALL(A->q1tab.alloc(tab.size()));
     DOHP_HIP(hipMemcpyAsync(A->q1tab.ptr, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, st));
     DOHP_HIP(hipStreamSynchronize(st));  // the host vector goes out of scope
   }

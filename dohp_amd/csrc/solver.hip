@@ -137,12 +137,12 @@ template <typename T>
 struct Ops;
 template <>
 struct Ops<double> {
-  static dErr A(dohpEllip op, const double *x, double *y, hipStream_t s) { return dohpEllipShellMatMult(op, x, y, s); }
+  static dErr A(dohpEllip op, const double *x, double *y, hipStream_t s) { return dohpEllipMatMult_Private(op, x, y, s); }
   static dErr Q(dohpMat m, const double *x, double *y, hipStream_t s) { return dohpMatMult(m, x, y, s); }
 };
 template <>
 struct Ops<float> {
-  static dErr A(dohpEllip op, const float *x, float *y, hipStream_t s) { return dohpEllipShellMatMultF32(op, x, y, s); }
+  static dErr A(dohpEllip op, const float *x, float *y, hipStream_t s) { return dohpEllipMatMult_Private(op, x, y, s); }
   static dErr Q(dohpMat m, const float *x, float *y, hipStream_t s) { return dohpMatMultF32(m, x, y, s); }
 };
 
@@ -282,6 +282,7 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double
   DOHP_CALL(dot(c, b, b, &bb));
   DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
   *bnorm = std::sqrt(bb);
+  DOHP_CHECK(std::isfinite(bb) && std::isfinite(rr), dohpERR_NAN, "PCG: non-finite right-hand side or initial residual");
   const double tol = rtol * std::sqrt(bb);
   int k = 0, ns = 0;
   double rz = 0;
@@ -309,6 +310,7 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double
     DOHP_CALL(axpby<T>(c, (T)alpha, p.ptr, T(1), x));
     DOHP_CALL(axpby<T>(c, (T)-alpha, q.ptr, T(1), r.ptr));
     DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
+    DOHP_CHECK(std::isfinite(rr), dohpERR_NAN, "PCG: non-finite residual at iteration " + std::to_string(k + 1));
     k++;
     bud.used++;
     if (snap && every > 0 && k % every == 0 && ns < nsnap) {
@@ -473,12 +475,13 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     int outer = 0, inner_total = 0;
     for (;;) {
       // FP64 defect r = b - A64 x
-      DOHP_CALL(dohpEllipShellMatMult(op, x, t.ptr, c.st));
+      DOHP_CALL(dohpEllipMatMult_Private(op, x, t.ptr, c.st));
       DOHP_HIP(hipMemcpyAsync(r.ptr, b, sizeof(double) * c.n, hipMemcpyDeviceToDevice, c.st));
       DOHP_CALL(axpby<double>(c, -1.0, t.ptr, 1.0, r.ptr));
       double rr;
       DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
       stats->rnorm = std::sqrt(rr);
+      DOHP_CHECK(std::isfinite(rr), dohpERR_NAN, "defect correction: non-finite FP64 residual");
       if (stats->rnorm <= opts->rtol * stats->bnorm || outer >= opts->max_outer) break;
       // FP32 correction, normalized so the inner problem is O(1)
       const double s = 1.0 / stats->rnorm;

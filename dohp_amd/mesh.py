@@ -87,3 +87,30 @@ def element_vertex_index(M) -> np.ndarray:
 def uniform_integers(seed: int, count: int, lo: int, hi: int) -> np.ndarray:
     """Seeded integers uniform on lo..hi (inclusive) from the splitmix64 stream."""
     return lo + np.minimum((splitmix64_uniform(seed, count) * (hi - lo + 1)).astype(np.int64), hi - lo)
+
+
+def sine_modes_state(X, seed: int) -> np.ndarray:
+    """u0 = sum over 8 modes of 0.1 sin(2 pi k_m . x + phi_m) at the points X [n][3], with k_m in {1..4}^3 and
+    phi_m uniform in [0, 2 pi), both drawn from the splitmix64 stream of `seed` (the linearization state of the
+    order-6 nonlinear diffusion workload)."""
+    r = splitmix64_uniform(seed, 32)
+    u = np.zeros(len(X))
+    for m in range(8):
+        k = 1 + np.floor(r[4 * m:4 * m + 3] * 4)
+        u += 0.1 * np.sin(2 * np.pi * (X @ k) + 2 * np.pi * r[4 * m + 3])
+    return u
+
+
+def mixed_order_batch(Me: int = 32, nprob: int = 256):
+    """The batched hp workload: an Me^3 perturbed mesh on [0,1]^3 whose elements take degrees uniform on 2..10
+    (seeded), plus nprob independent Poisson problems on n^3 perturbed meshes, n uniform on 4..16, degree uniform
+    on 2..8.  Returns (degrees [Me^3], element vertices [Me^3][8][3], [(M, degree, V)] of the problems)."""
+    M = (Me, Me, Me)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=3)
+    deg = uniform_integers(4, Me ** 3, 2, 10)
+    ns, ps = uniform_integers(6, nprob, 4, 16), uniform_integers(7, nprob, 2, 8)
+    probs = []
+    for i in range(nprob):
+        Mi = (int(ns[i]),) * 3
+        probs.append((Mi, int(ps[i]), block_vertices(Mi, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=1000 + i)))
+    return deg, element_vertices(M, V), probs

@@ -34,7 +34,10 @@ enum { dGAUSS_LEGENDRE = 0, dGAUSS_LOBATTO = 1 };
 /* Precision of an operator application. */
 enum { dohpPREC_F64 = 0, dohpPREC_F32 = 1 };
 /* Error codes beyond generic failure. */
-enum { dohpERR_GENERIC = 1, dohpERR_ARG = 62, dohpERR_NEGATIVE_JACOBIAN = 77, dohpERR_HIP = 99 };
+/* dohpERR_NAN: a non-finite value met on the device (input vector, state, result); dohpERR_OOM: a device
+ * allocation failed (the object stays usable: a later call allocates again). */
+enum { dohpERR_GENERIC = 1, dohpERR_ARG = 62, dohpERR_NEGATIVE_JACOBIAN = 77, dohpERR_NAN = 78, dohpERR_OOM = 98,
+       dohpERR_HIP = 99 };
 
 /* Copies the message of the last error on this thread into buf. */
 void dohpGetLastError(char *buf, size_t len);
@@ -58,6 +61,16 @@ dErr dohpTensorBasis(int P, int Q, const double *rcoord_host, double *interp_hos
  * [nelem][Q^3][9] or NULL (GRAD only: layout swap, efstopo.c:518-546).  rcoord is the 1-D rule. */
 dErr dEFSApplyHexBatched(int P, int Q, const double *rcoord_host, int nelem, const double *jinv_dev, int D,
                          const double *in_dev, double *out_dev, int amode, int imode, void *stream);
+
+/* Batched dRuleComputePhysical (include/dohpjacobi.h:193, src/jacobi/interface/jacobi.c:383-414) for a
+ * tensor rule of Q points per direction, 1-D weights weight_host[Q], point order (i Q + j) Q + k
+ * (jacobi.c:351): for each of nelem elements and each point q, jinv[q] = jac[q]^-1 (3x3 row-major
+ * cofactor inverse, dGeomInvert3, include/dohpgeom.h:57-75) and jw[q] = w_i w_j w_k det jac[q].
+ * jac_dev and jinv_dev are [nelem][Q^3][9], jw_dev [nelem][Q^3].  A negative determinant returns
+ * dohpERR_NEGATIVE_JACOBIAN (jacobi.c:397); a non-finite one returns dohpERR_NAN (the reference
+ * carries it on).  Waits for the stream (the device-detected flag is read back). */
+dErr dRuleComputePhysicalBatched(int Q, const double *weight_host, int nelem, const double *jac_dev,
+                                 double *jinv_dev, double *jw_dev, void *stream);
 
 /* ---------------------------------------------------------------------- dFS */
 typedef struct _p_dohpFS *dohpFS;
@@ -104,12 +117,19 @@ dErr dohpEllipDestroy(dohpEllip *op);
  * diffusion: -div(k eta grad u).  NULL restores k = 1. */
 dErr dohpEllipSetCoefficient(dohpEllip op, const double *kvert_host);
 /* Residual F(u) (EllipFunction, src/fs/tests/ellip.c:407-440) with inhomogeneous Dirichlet values;
- * records u as the linearization state of later Jacobian applications (the store of ellip.c:311). */
+ * records u as the linearization state of later Jacobian applications (the store of ellip.c:311).
+ * With checks on (dohpEllipSetChecks, default) the residual and the MatMult entry points wait for their
+ * stream and return dohpERR_NAN when the element kernels met a non-finite contribution (a NaN or Inf in
+ * the input vector or the state). */
 dErr dohpEllipFunction(dohpEllip op, const double *u_dev, double *f_dev, void *stream);
 /* y = J(u) x, homogeneous Dirichlet (EllipShellMatMult, src/fs/tests/ellip.c:442-472): the MATSHELL
  * MATOP_MULT body.  FP64 and FP32 variants; the FP32 one keeps geometry in FP64. */
 dErr dohpEllipShellMatMult(dohpEllip op, const double *x_dev, double *y_dev, void *stream);
 dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x_dev, float *y_dev, void *stream);
+/* 1 (default): the residual and MatMult entry points wait for their stream and report non-finite values
+ * (dohpERR_NAN); 0: they return at once, unchecked (asynchronous use; the library's solvers apply the
+ * operator unchecked and test the FP64 residual norm instead, returning dohpERR_NAN if it is not finite). */
+dErr dohpEllipSetChecks(dohpEllip op, int mode);
 /* Scatter strategy of the element kernels: 0 = eight-colour deterministic (bitwise reproducible),
  * 1 = hardware atomics (default; summation order varies at rounding level). */
 dErr dohpEllipSetScatter(dohpEllip op, int mode);

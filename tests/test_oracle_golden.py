@@ -321,3 +321,25 @@ def test_ellip_morph_linear_solve_error_norms():
     ref = HIST["ellip-morph"]["error_norms"]
     np.testing.assert_allclose(en, ref["solution"], rtol=5e-3)
     np.testing.assert_allclose(gn, ref["gradient"], rtol=5e-3)
+
+
+def test_rule_compute_physical_closed_form():
+    """orc_rule_compute_physical (jacobi.c:383-414 with dGeomInvert3, dohpgeom.h:57-75) against numpy's inverse
+    and determinant of non-symmetric matrices (a transposed or sign-flipped cofactor fails), jw = w det, and the
+    negative-determinant error."""
+    import ctypes
+    rng = np.random.default_rng(7)
+    n = 40
+    jac = rng.uniform(-1, 1, (n, 3, 3)) + 2.5 * np.eye(3)
+    w = rng.uniform(0.1, 1.0, n)
+    jinv, jw = np.zeros((n, 3, 3)), np.zeros(n)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = O.lib().orc_rule_compute_physical(n, w.ctypes.data_as(dp), jac.ctypes.data_as(dp), jinv.ctypes.data_as(dp),
+                                           jw.ctypes.data_as(dp))
+    assert rc == 0
+    np.testing.assert_allclose(jinv, np.linalg.inv(jac), rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(jw, w * np.linalg.det(jac), rtol=1e-13)
+    bad = jac.copy()
+    bad[5] = -bad[5]  # det < 0
+    assert O.lib().orc_rule_compute_physical(n, w.ctypes.data_as(dp), bad.ctypes.data_as(dp), jinv.ctypes.data_as(dp),
+                                             jw.ctypes.data_as(dp)) == 2

@@ -50,6 +50,8 @@ _proto("dohpTensorRule", ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp
 _proto("dohpTensorBasis", ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp)
 _proto("dEFSApplyHexBatched", ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp,
        ctypes.c_int, ctypes.c_int, _vp)
+_proto("dEFSApplyHexBatchedNoAlloc", ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp,
+       ctypes.c_int, ctypes.c_int, _vp)
 _proto("dRuleComputePhysicalBatched", ctypes.c_int, _dp, ctypes.c_int, _vp, _vp, _vp, _vp)
 _proto("dFSCreateBlock", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(_vp))
 _proto("dFSDestroy", ctypes.POINTER(_vp))
@@ -283,6 +285,14 @@ def efs_apply_hex_batched(P, rcoord, nelem, jinv, D, x, y, amode, imode=INSERT_V
     rc, rcp = _np(rcoord)
     check(_lib.dEFSApplyHexBatched(P, rc.size, rcp, nelem, _ptr(jinv), D, _ptr(x), _ptr(y), amode, imode,
                                    _stream(stream)))
+
+
+def efs_apply_hex_batched_noalloc(P, rcoord, nelem, jinv, D, x, y, amode, imode=INSERT_VALUES, stream=None):
+    """dEFSApplyHexBatched without device allocation (tables staged in the library's per-GPU array); returns
+    without waiting for the stream."""
+    rc, rcp = _np(rcoord)
+    check(_lib.dEFSApplyHexBatchedNoAlloc(P, rc.size, rcp, nelem, _ptr(jinv), D, _ptr(x), _ptr(y), amode, imode,
+                                          _stream(stream)))
 
 
 def rule_compute_physical_batched(weight1d, nelem, jac, jinv, jw, stream=None):

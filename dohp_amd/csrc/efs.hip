@@ -4,6 +4,7 @@
 // efstopo.c:518-623).  One workgroup per element; each 1-D contraction is one pass over
 // an LDS tile, components of a block size D > 1 are processed one at a time.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -210,6 +211,38 @@ dErr efs_launch(int P, int Q, const double *dtab, int nelem, const double *jinv,
   return 0;
 }
 
+// The 1-D tables of the allocation-free entry point: one copy per GPU, written by a kernel that takes them by
+// value (no copy from host memory), read by the apply kernel that follows on the same stream.
+static __device__ __align__(256) double scratch[kEfsTab];
+
+struct EfsTab {
+  double v[kEfsTab];
+};
+
+__global__ void efs_store_tab_kernel(EfsTab t, int n, double *__restrict__ dst)
+{
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = t.v[i];
+}
+
+// Device address of `scratch` on the current device, looked up once per device (the host-side name of a
+// __device__ array is not its device address).
+dErr efs_scratch(double **ptr)
+{
+  static std::mutex mu;
+  static double *addr[64] = {};
+  int dev = 0;
+  DOHP_HIP(hipGetDevice(&dev));
+  DOHP_CHECK(dev >= 0 && dev < 64, dohpERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    DOHP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    addr[dev] = (double *)p;
+  }
+  *ptr = addr[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" dErr dEFSApplyHexBatched(int P, int Q, const double *rcoord, int nelem, const double *jinv, int D,
@@ -225,4 +258,20 @@ extern "C" dErr dEFSApplyHexBatched(int P, int Q, const double *rcoord, int nele
   DOHP_CALL(efs_launch(P, Q, dtab.ptr, nelem, jinv, D, in, out, amode, imode, st));
   DOHP_HIP(hipStreamSynchronize(st));  // dtab is freed on return
   return 0;
+}
+
+extern "C" dErr dEFSApplyHexBatchedNoAlloc(int P, int Q, const double *rcoord, int nelem, const double *jinv, int D,
+                                           const double *in, double *out, int amode, int imode, void *stream)
+{
+  DOHP_CALL(efs_check(P, Q, nelem, rcoord, jinv, D, in, out, amode, imode));
+  DOHP_CHECK(2 * Q * P <= kEfsTab, dohpERR_ARG, "dEFSApplyHexBatchedNoAlloc: tables exceed the 2*12*12 staging area");
+  if (nelem == 0) return 0;
+  double *dtab = nullptr;
+  DOHP_CALL(efs_scratch(&dtab));
+  EfsTab t;  // host stack: B and D of the rule, passed to the device by value
+  DOHP_CALL(dohpTensorBasis(P, Q, rcoord, t.v, t.v + Q * P, nullptr, nullptr));
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(efs_store_tab_kernel, dim3(1), dim3(256), 0, st, t, 2 * Q * P, dtab);
+  DOHP_HIP(hipGetLastError());
+  return efs_launch(P, Q, dtab, nelem, jinv, D, in, out, amode, imode, st);
 }

@@ -62,6 +62,15 @@ dErr dohpTensorBasis(int P, int Q, const double *rcoord_host, double *interp_hos
 dErr dEFSApplyHexBatched(int P, int Q, const double *rcoord_host, int nelem, const double *jinv_dev, int D,
                          const double *in_dev, double *out_dev, int amode, int imode, void *stream);
 
+/* No counterpart in the reference.  dEFSApplyHexBatched without any device allocation or wait, for callers
+ * that cannot allocate (code under HIP graph capture, tight loops applying elements the way the reference's
+ * per-element dEFSApply call sites do).  Same arguments, results and errors as dEFSApplyHexBatched (it shares its
+ * kernel); P, Q <= 12.  The B and D tables (2 Q P <= 288 doubles) are staged in a 2.3 KB device array of the
+ * library, one per GPU, held until the process ends: calls that overlap on one GPU (several streams or host
+ * threads) race on it. */
+dErr dEFSApplyHexBatchedNoAlloc(int P, int Q, const double *rcoord_host, int nelem, const double *jinv_dev, int D,
+                                const double *in_dev, double *out_dev, int amode, int imode, void *stream);
+
 /* Batched dRuleComputePhysical (include/dohpjacobi.h:193, src/jacobi/interface/jacobi.c:383-414) for a
  * tensor rule of Q points per direction, 1-D weights weight_host[Q], point order (i Q + j) Q + k
  * (jacobi.c:351): for each of nelem elements and each point q, jinv[q] = jac[q]^-1 (3x3 row-major

@@ -157,6 +157,17 @@ _proto("dohpMLPGradient", _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp)
 _proto("dohpMLPEnergyWeightGradient", _vp, _vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp)
 _proto("dohpMLPWeightStep", _vp, _vp, _vp, _vp, ctypes.c_float, _vp)
 _proto("dohpMLPEnergyStep", _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_float, _vp, _vp)
+for _f in ("dohpMLPNumParamsKind", "dohpMLPNumInputsKind", "dohpMLPNumOutputsKind"):
+    getattr(_lib, _f).argtypes = [ctypes.c_int]
+    getattr(_lib, _f).restype = ctypes.c_int
+_proto("dohpMLPCreateKind", ctypes.c_int, _vp, _vp)
+_proto("dohpMLPCellScales", _vp, ctypes.c_long, _vp, _vp, _vp)
+_proto("dohpMLPCellStep", _vp, ctypes.c_long, _vp, _vp, ctypes.c_float, _vp)
+_proto("dohpEllipSetQ1CellScales", _vp, _vp)
+_proto("dohpQ1CellCount", _vp, _vp)
+_proto("dohpQ1CellFeatures", _vp, _vp, _vp, _vp)
+_proto("dohpQ1CellScaleGradient", _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp)
+_proto("dohpEllipSolveFallback", _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)
 
 
 class MLP:
@@ -221,6 +232,49 @@ class MLP:
         check(_lib.dohpMLPEnergyStep(self.h, op.h, q1.h, mg.h, _ptr(X), _ptr(r), _ptr(e), ctypes.c_double(alpha),
                                      ctypes.c_float(lr), ctypes.byref(E), _stream(stream)))
         return E.value
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.dohpMLPDestroy(ctypes.byref(self.h))
+
+
+class CellMLP:
+    """Learned sub-cell derivative scales of the embedded-Q1 matrix (dohpMLP of kind 1): the features of each GLL
+    sub-cell map to sigma = exp(clamp(o)) per reference direction."""
+
+    kind = 1
+    nparam = _lib.dohpMLPNumParamsKind(1)
+    ninputs = _lib.dohpMLPNumInputsKind(1)
+    noutputs = _lib.dohpMLPNumOutputsKind(1)
+
+    def __init__(self, params=None):
+        self.h = ctypes.c_void_p()
+        p = None
+        if params is not None:
+            self._p = np.ascontiguousarray(params, dtype=np.float32)
+            assert self._p.size == self.nparam
+            p = self._p.ctypes.data_as(ctypes.c_void_p)
+        check(_lib.dohpMLPCreateKind(1, p, ctypes.byref(self.h)))
+
+    def params(self):
+        m, w = np.zeros(self.nparam, np.float32), np.zeros(self.nparam, np.float32)
+        check(_lib.dohpMLPGetParams(self.h, m.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(ctypes.c_void_p)))
+        return m, w
+
+    def gradient(self, X, T, B=None, stream=None):
+        """MSE gradient of the raw outputs (checks the training kernels)."""
+        B = int(B or X.shape[0])
+        g = np.zeros(self.nparam, np.float32)
+        loss = ctypes.c_double()
+        check(_lib.dohpMLPGradient(self.h, B, _ptr(X), _ptr(T), g.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss),
+                                   _stream(stream)))
+        return loss.value, g
+
+    def scales(self, X, sig, stream=None):
+        check(_lib.dohpMLPCellScales(self.h, int(X.shape[0]), _ptr(X), _ptr(sig), _stream(stream)))
+
+    def step(self, X, gsig, lr, stream=None):
+        check(_lib.dohpMLPCellStep(self.h, int(X.shape[0]), _ptr(X), _ptr(gsig), ctypes.c_float(lr), _stream(stream)))
 
     def __del__(self):
         if _lib is not None and getattr(self, "h", None):
@@ -445,6 +499,25 @@ class Ellip:
         non-finite value met the element kernels; 0: asynchronous, unchecked."""
         check(_lib.dohpEllipSetChecks(self.h, int(mode)))
 
+    def set_q1_cell_scales(self, sig):
+        """Learned sub-cell derivative scales [ncell][3] (device float32 tensor, kept alive here) for later Q1
+        assemblies; None = none."""
+        self._cscale = sig
+        check(_lib.dohpEllipSetQ1CellScales(self.h, _ptr(sig)))
+
+    def q1_cell_count(self):
+        n = ctypes.c_long()
+        check(_lib.dohpQ1CellCount(self.h, ctypes.byref(n)))
+        return n.value
+
+    def q1_cell_features(self, q1, X, stream=None):
+        check(_lib.dohpQ1CellFeatures(self.h, q1.h, _ptr(X), _stream(stream)))
+
+    def q1_cell_scale_gradient(self, q1, w, z, coef, g, stream=None):
+        """g[cell][a] += coef * w^T dK/dsigma_{cell,a} z (the scales set on this operator)."""
+        check(_lib.dohpQ1CellScaleGradient(self.h, q1.h, _ptr(w), _ptr(z), ctypes.c_double(coef), _ptr(g),
+                                           _stream(stream)))
+
     def set_q1_mode(self, mode):
         """Later embedded-Q1 assemblies: 0 = the reference's arithmetic, 1 = corrected (GLL sub-mesh Q1 FEM)."""
         check(_lib.dohpEllipSetQ1Mode(self.h, int(mode)))
@@ -471,17 +544,21 @@ class Ellip:
 
     def solve(self, q1, b, x, mixed=False, rtol=1e-10, max_it=2000, inner_rtol=1e-4, max_outer=50, cheb_degree=4,
               cheb_ratio=30.0, eig_its=20, pc="mg", mg_degree=0, mlp=None, fallback_its=0, record=None,
-              stream=None):
+              fallback_q1=None, stream=None):
         """CG on the linear operator, preconditioned through the embedded-Q1 matrix: pc="mg" is a
         multigrid V-cycle with Jacobi-Chebyshev smoothing, pc="cheb" Jacobi-Chebyshev alone.
         mixed: 0 plain FP64, 1 FP32 inner PCG inside FP64 defect correction, 2 FP64 PCG with the
         preconditioner in FP32.  mg_degree: smoothing degree, 0 = 1 on a corrected Q1 matrix, 2 otherwise.  mlp: learned node weights around the V-cycle (plain fallback after
-        fallback_its iterations).  record=(snap, every): plain FP64 solve keeping iterates."""
+        fallback_its iterations).  fallback_q1: q1 is a learned embedded-Q1 matrix and the multigrid of fallback_q1
+        takes over after fallback_its iterations.  record=(snap, every): plain FP64 solve keeping iterates."""
         o = SolveOpts(int(mixed), float(rtol), int(max_it), float(inner_rtol), int(max_outer), int(cheb_degree),
                       float(cheb_ratio), int(eig_its), {"cheb": 0, "mg": 1}[pc], int(mg_degree), int(fallback_its))
         st = SolveStats()
         out = {}
-        if mlp is not None:
+        if fallback_q1 is not None:
+            check(_lib.dohpEllipSolveFallback(self.h, q1.h, fallback_q1.h, _ptr(b), _ptr(x), ctypes.byref(o),
+                                              ctypes.byref(st), _stream(stream)))
+        elif mlp is not None:
             check(_lib.dohpEllipSolveLearned(self.h, q1.h, mlp.h, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(st),
                                              _stream(stream)))
         elif record is not None:

@@ -38,6 +38,8 @@ void set_error(const std::string &msg);
   } while (0)
 
 constexpr int kMaxN = 16;
+// Inputs per GLL sub-cell of the learned embedded-Q1 scales (features kernel in q1.hip, network in learn.hip).
+constexpr int kQ1CellInputs = 11;
 
 // 1-D rules and bases (host).
 int gauss_legendre(int n, double *z, double *w);

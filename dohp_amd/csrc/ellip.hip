@@ -393,6 +393,7 @@ struct _p_dohpEllip {
   DevBuf<int> nanflag;
   int *nan_host = nullptr;  // pinned
   int checks = 1;
+  const float *q1_cscale = nullptr;  // learned sub-cell derivative scales of later Q1 assemblies (caller's array)
   // host tables
   std::vector<double> B, Dh, w, phi;
   ~_p_dohpEllip()
@@ -844,6 +845,15 @@ extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpElli
   *kvert = op->kvert.ptr;
   *state = op->state.ptr;
   if (q1mode) *q1mode = op->q1mode;
+  return 0;
+}
+
+const float *dohpEllipGetQ1CellScales_Private(dohpEllip op) { return op->q1_cscale; }
+
+extern "C" dErr dohpEllipSetQ1CellScales(dohpEllip op, const float *scales_dev)
+{
+  DOHP_CHECK(op, dohpERR_ARG, "null operator");
+  op->q1_cscale = scales_dev;
   return 0;
 }
 

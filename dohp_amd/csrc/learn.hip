@@ -29,10 +29,24 @@ using namespace dohp;
 
 namespace {
 
-constexpr int NIN = 32, NH = 64, NOUT = 8;
-constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NIN, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
-              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NOUT * NH, NPARAM = OFF_B3 + NOUT;
-constexpr float kClamp = 4.0f;  // |log weight| limit
+constexpr int NH = 64;
+// Parameter layout of a network with NIN_ inputs and NOUT_ outputs: [W1 NH x NIN | b1 | W2 NH x NH | b2 | W3 NOUT x NH | b3].
+template <int NIN_, int NOUT_>
+struct Shape {
+  static constexpr int NIN = NIN_, NOUT = NOUT_;
+  static constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NIN, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+                       OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NOUT * NH, NPARAM = OFF_B3 + NOUT;
+};
+// Network kinds: 0 = node weights around the V-cycle (32 patch features -> 8 log-weights of the patch nodes);
+// 1 = sub-cell derivative scales of the embedded-Q1 matrix (11 sub-cell features -> 3 log-scales).
+using PatchNet = Shape<32, 8>;
+using CellNet = Shape<kQ1CellInputs, 3>;
+constexpr int NIN = PatchNet::NIN, NOUT = PatchNet::NOUT;
+constexpr int NPARAM_MAX = PatchNet::NPARAM > CellNet::NPARAM ? PatchNet::NPARAM : CellNet::NPARAM;
+inline int nparam_of(int kind) { return kind == 1 ? CellNet::NPARAM : PatchNet::NPARAM; }
+inline int nin_of(int kind) { return kind == 1 ? CellNet::NIN : PatchNet::NIN; }
+inline int nout_of(int kind) { return kind == 1 ? CellNet::NOUT : PatchNet::NOUT; }
+constexpr float kClamp = 4.0f;  // |log weight| and |log scale| limit
 
 struct PBox {
   int nx, ny, nz;  // node box
@@ -66,32 +80,33 @@ __device__ __forceinline__ float dsilu(float x)
 // Weights are read as scalar operands (uniform over the wavefront) from the constant space.
 typedef const __attribute__((address_space(4))) float CW;
 
-__device__ __forceinline__ void mlp_forward(CW *w, const float (&f)[NIN], float (&z1)[NH], float (&z2)[NH],
-                                            float (&o)[NOUT])
+template <class S>
+__device__ __forceinline__ void mlp_forward(CW *w, const float (&f)[S::NIN], float (&z1)[NH], float (&z2)[NH],
+                                            float (&o)[S::NOUT])
 {
   float h1[NH];
 #pragma unroll
   for (int i = 0; i < NH; i++) {
-    float a = w[OFF_B1 + i];
+    float a = w[S::OFF_B1 + i];
 #pragma unroll
-    for (int j = 0; j < NIN; j++) a = fmaf(w[OFF_W1 + i * NIN + j], f[j], a);
+    for (int j = 0; j < S::NIN; j++) a = fmaf(w[S::OFF_W1 + i * S::NIN + j], f[j], a);
     z1[i] = a;
     h1[i] = silu(a);
   }
   float h2[NH];
 #pragma unroll
   for (int i = 0; i < NH; i++) {
-    float a = w[OFF_B2 + i];
+    float a = w[S::OFF_B2 + i];
 #pragma unroll
-    for (int j = 0; j < NH; j++) a = fmaf(w[OFF_W2 + i * NH + j], h1[j], a);
+    for (int j = 0; j < NH; j++) a = fmaf(w[S::OFF_W2 + i * NH + j], h1[j], a);
     z2[i] = a;
     h2[i] = silu(a);
   }
 #pragma unroll
-  for (int i = 0; i < NOUT; i++) {
-    float a = w[OFF_B3 + i];
+  for (int i = 0; i < S::NOUT; i++) {
+    float a = w[S::OFF_B3 + i];
 #pragma unroll
-    for (int j = 0; j < NH; j++) a = fmaf(w[OFF_W3 + i * NH + j], h2[j], a);
+    for (int j = 0; j < NH; j++) a = fmaf(w[S::OFF_W3 + i * NH + j], h2[j], a);
     o[i] = a;
   }
 }
@@ -122,15 +137,17 @@ __global__ void features_kernel(PBox b, int p, int lo0, int lo1, int lo2, const 
 }
 
 // Forward over B samples, keeping the pre-activations for training (Z1, Z2 may be null).
+template <class S>
 __global__ void forward_kernel(int B, const float *__restrict__ wts, const float *__restrict__ X,
                                float *__restrict__ Z1, float *__restrict__ Z2, float *__restrict__ O)
 {
+  constexpr int NIN = S::NIN, NOUT = S::NOUT;
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B) return;
   float f[NIN], z1[NH], z2[NH], o[NOUT];
 #pragma unroll
   for (int j = 0; j < NIN; j++) f[j] = X[s * NIN + j];
-  mlp_forward((CW *)wts, f, z1, z2, o);
+  mlp_forward<S>((CW *)wts, f, z1, z2, o);
   if (Z1) {
 #pragma unroll
     for (int i = 0; i < NH; i++) {
@@ -177,9 +194,11 @@ __global__ void dweights_kernel(PBox b, const float *__restrict__ O, const doubl
 }
 
 // Squared-error loss against targets (the gradient check of the training kernels).
+template <class S>
 __global__ void mse_grad_kernel(int B, const float *__restrict__ O, const float *__restrict__ Tg, float *__restrict__ dO,
                                 float *__restrict__ sqerr)
 {
+  constexpr int NOUT = S::NOUT;
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B) return;
   float e2 = 0.f;
@@ -191,10 +210,12 @@ __global__ void mse_grad_kernel(int B, const float *__restrict__ O, const float 
   sqerr[s] = e2;
 }
 
+template <class S>
 __global__ void backward_kernel(int B, const float *__restrict__ wts, const float *__restrict__ Z1,
                                 const float *__restrict__ Z2, const float *__restrict__ dO, float *__restrict__ dZ1,
                                 float *__restrict__ dZ2)
 {
+  constexpr int NOUT = S::NOUT, OFF_W3 = S::OFF_W3, OFF_W2 = S::OFF_W2;
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B) return;
   CW *w = (CW *)wts;
@@ -214,10 +235,13 @@ __global__ void backward_kernel(int B, const float *__restrict__ wts, const floa
 }
 
 // Parameter gradient, one thread per parameter, summing over samples in order (deterministic).
+template <class S>
 __global__ void grad_kernel(int B, const float *__restrict__ X, const float *__restrict__ Z1,
                             const float *__restrict__ Z2, const float *__restrict__ dO, const float *__restrict__ dZ1,
                             const float *__restrict__ dZ2, float *__restrict__ grad)
 {
+  constexpr int NIN = S::NIN, NOUT = S::NOUT, NPARAM = S::NPARAM, OFF_B1 = S::OFF_B1, OFF_W2 = S::OFF_W2,
+                OFF_B2 = S::OFF_B2, OFF_W3 = S::OFF_W3, OFF_B3 = S::OFF_B3;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= NPARAM) return;
   float acc = 0.f;
@@ -251,12 +275,12 @@ __device__ __forceinline__ float to_bf16_value(float x)
 }
 
 // Adam on the FP32 master weights; the network uses their bf16 values.
-__global__ void adam_kernel(int t, float lr, float b1, float b2, float eps, const float *__restrict__ grad,
+__global__ void adam_kernel(int nparam, int t, float lr, float b1, float b2, float eps, const float *__restrict__ grad,
                             float *__restrict__ master, float *__restrict__ m, float *__restrict__ v,
                             float *__restrict__ wts)
 {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= NPARAM) return;
+  if (q >= nparam) return;
   const float g = grad[q];
   const float mq = b1 * m[q] + (1.f - b1) * g, vq = b2 * v[q] + (1.f - b2) * g * g;
   m[q] = mq;
@@ -300,6 +324,22 @@ __global__ void dot_kernel(long n, const double *__restrict__ a, const double *_
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// Sub-cell derivative scales sigma = exp(clamp(o)) from the cell network's outputs.
+__global__ void cell_scales_kernel(long ncell, const float *__restrict__ O, float *__restrict__ sig)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * ncell) sig[i] = __expf(fminf(fmaxf(O[i], -kClamp), kClamp));
+}
+// dL/do = dL/dsigma * sigma inside the clamp, 0 outside.
+__global__ void dcell_scales_kernel(long ncell, const float *__restrict__ O, const float *__restrict__ gsig,
+                                    float *__restrict__ dO)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * ncell) return;
+  const float o = O[i];
+  dO[i] = (o > -kClamp && o < kClamp) ? gsig[i] * __expf(o) : 0.f;
+}
+
 inline dim3 grid_of(long n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
 float host_bf16(float x)
@@ -316,6 +356,7 @@ float host_bf16(float x)
 }  // namespace
 
 struct _p_dohpMLP {
+  int kind = 0;  // 0: patch node weights, 1: embedded-Q1 sub-cell scales
   DevBuf<float> wts, master, m, v, grad;
   int step = 0;
   long B = 0;  // capacity of the per-sample buffers
@@ -332,18 +373,43 @@ dErr ensure(_p_dohpMLP *n, long B)
   DOHP_CALL(n->Z2.alloc((size_t)B * NH));
   DOHP_CALL(n->dZ1.alloc((size_t)B * NH));
   DOHP_CALL(n->dZ2.alloc((size_t)B * NH));
-  DOHP_CALL(n->O.alloc((size_t)B * NOUT));
-  DOHP_CALL(n->dO.alloc((size_t)B * NOUT));
+  DOHP_CALL(n->O.alloc((size_t)B * nout_of(n->kind)));
+  DOHP_CALL(n->dO.alloc((size_t)B * nout_of(n->kind)));
   DOHP_CALL(n->sqerr.alloc(B));
   return 0;
 }
 
+template <class S>
+dErr backward_and_grad_s(_p_dohpMLP *n, long B, const float *X, hipStream_t st)
+{
+  hipLaunchKernelGGL(backward_kernel<S>, grid_of(B, 128), dim3(128), 0, st, (int)B, n->wts.ptr, n->Z1.ptr, n->Z2.ptr,
+                     n->dO.ptr, n->dZ1.ptr, n->dZ2.ptr);
+  hipLaunchKernelGGL(grad_kernel<S>, grid_of(S::NPARAM), dim3(256), 0, st, (int)B, X, n->Z1.ptr, n->Z2.ptr, n->dO.ptr,
+                     n->dZ1.ptr, n->dZ2.ptr, n->grad.ptr);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
 dErr backward_and_grad(_p_dohpMLP *n, long B, const float *X, hipStream_t st)
 {
-  hipLaunchKernelGGL(backward_kernel, grid_of(B, 128), dim3(128), 0, st, (int)B, n->wts.ptr, n->Z1.ptr, n->Z2.ptr,
-                     n->dO.ptr, n->dZ1.ptr, n->dZ2.ptr);
-  hipLaunchKernelGGL(grad_kernel, grid_of(NPARAM), dim3(256), 0, st, (int)B, X, n->Z1.ptr, n->Z2.ptr, n->dO.ptr,
-                     n->dZ1.ptr, n->dZ2.ptr, n->grad.ptr);
+  return n->kind == 1 ? backward_and_grad_s<CellNet>(n, B, X, st) : backward_and_grad_s<PatchNet>(n, B, X, st);
+}
+dErr forward(_p_dohpMLP *n, long B, const float *X, hipStream_t st)
+{
+  if (n->kind == 1)
+    hipLaunchKernelGGL(forward_kernel<CellNet>, grid_of(B, 128), dim3(128), 0, st, (int)B, n->wts.ptr, X, n->Z1.ptr,
+                       n->Z2.ptr, n->O.ptr);
+  else
+    hipLaunchKernelGGL(forward_kernel<PatchNet>, grid_of(B, 128), dim3(128), 0, st, (int)B, n->wts.ptr, X, n->Z1.ptr,
+                       n->Z2.ptr, n->O.ptr);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+dErr adam_step(_p_dohpMLP *n, float lr, hipStream_t st)
+{
+  n->step++;
+  const int np = nparam_of(n->kind);
+  hipLaunchKernelGGL(adam_kernel, grid_of(np), dim3(256), 0, st, np, n->step, lr, 0.9f, 0.999f, 1e-8f, n->grad.ptr,
+                     n->master.ptr, n->m.ptr, n->v.ptr, n->wts.ptr);
   DOHP_HIP(hipGetLastError());
   return 0;
 }
@@ -409,14 +475,22 @@ dErr energy_and_grad(dohpEllip op, dohpMG mg, long nn, const double *s, const do
 
 }  // namespace
 
-extern "C" int dohpMLPNumParams(void) { return NPARAM; }
-extern "C" int dohpMLPNumInputs(void) { return NIN; }
+extern "C" int dohpMLPNumParams(void) { return PatchNet::NPARAM; }
+extern "C" int dohpMLPNumInputs(void) { return PatchNet::NIN; }
+extern "C" int dohpMLPNumParamsKind(int kind) { return kind == 0 || kind == 1 ? nparam_of(kind) : -1; }
+extern "C" int dohpMLPNumInputsKind(int kind) { return kind == 0 || kind == 1 ? nin_of(kind) : -1; }
+extern "C" int dohpMLPNumOutputsKind(int kind) { return kind == 0 || kind == 1 ? nout_of(kind) : -1; }
 
-extern "C" dErr dohpMLPCreate(const float *params_host, dohpMLP *out)
+extern "C" dErr dohpMLPCreate(const float *params_host, dohpMLP *out) { return dohpMLPCreateKind(0, params_host, out); }
+
+extern "C" dErr dohpMLPCreateKind(int kind, const float *params_host, dohpMLP *out)
 {
   DOHP_CHECK(out, dohpERR_ARG, "null output");
+  DOHP_CHECK(kind == 0 || kind == 1, dohpERR_ARG, "network kind must be 0 (patch node weights) or 1 (sub-cell scales)");
   *out = nullptr;
   _p_dohpMLP *n = new _p_dohpMLP;
+  n->kind = kind;
+  const int NPARAM = nparam_of(kind);
   std::vector<float> w(NPARAM, 0.0f), wb(NPARAM);
   if (params_host) std::memcpy(w.data(), params_host, sizeof(float) * NPARAM);
   for (int q = 0; q < NPARAM; q++) wb[q] = host_bf16(w[q]);
@@ -437,6 +511,7 @@ extern "C" dErr dohpMLPCreate(const float *params_host, dohpMLP *out)
 extern "C" dErr dohpMLPGetParams(dohpMLP n, float *master_host, float *bf16_values_host)
 {
   DOHP_CHECK(n, dohpERR_ARG, "null network");
+  const int NPARAM = nparam_of(n->kind);
   DOHP_HIP(hipDeviceSynchronize());
   if (master_host) DOHP_HIP(hipMemcpy(master_host, n->master.ptr, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
   if (bf16_values_host) DOHP_HIP(hipMemcpy(bf16_values_host, n->wts.ptr, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
@@ -480,12 +555,11 @@ extern "C" dErr dohpMLPWeights(dohpMLP n, dohpMat q1, const float *X_dev, double
   PBox b;
   DOHP_CALL(box_of(q1, &b));
   DOHP_CHECK(n && X_dev && s_dev, dohpERR_ARG, "null argument");
+  DOHP_CHECK(n->kind == 0, dohpERR_ARG, "node weights need a patch network (kind 0)");
   hipStream_t st = (hipStream_t)stream;
   const long np = b.npatch(), nn = (long)b.nx * b.ny * b.nz;
   DOHP_CALL(ensure(n, np > 0 ? np : 1));
-  if (np > 0)
-    hipLaunchKernelGGL(forward_kernel, grid_of(np, 128), dim3(128), 0, st, (int)np, n->wts.ptr, X_dev, n->Z1.ptr,
-                       n->Z2.ptr, n->O.ptr);
+  if (np > 0) DOHP_CALL(forward(n, np, X_dev, st));
   hipLaunchKernelGGL(weights_kernel, grid_of(nn), dim3(256), 0, st, b, n->O.ptr, s_dev, s32_dev);
   DOHP_HIP(hipGetLastError());
   return 0;
@@ -497,18 +571,23 @@ extern "C" dErr dohpMLPGradient(dohpMLP n, int B, const float *X_dev, const floa
   DOHP_CHECK(n && B > 0 && X_dev && T_dev, dohpERR_ARG, "bad argument");
   hipStream_t st = (hipStream_t)stream;
   DOHP_CALL(ensure(n, B));
-  hipLaunchKernelGGL(forward_kernel, grid_of(B, 128), dim3(128), 0, st, B, n->wts.ptr, X_dev, n->Z1.ptr, n->Z2.ptr,
-                     n->O.ptr);
-  hipLaunchKernelGGL(mse_grad_kernel, grid_of(B, 128), dim3(128), 0, st, B, n->O.ptr, T_dev, n->dO.ptr, n->sqerr.ptr);
+  DOHP_CALL(forward(n, B, X_dev, st));
+  if (n->kind == 1)
+    hipLaunchKernelGGL(mse_grad_kernel<CellNet>, grid_of(B, 128), dim3(128), 0, st, B, n->O.ptr, T_dev, n->dO.ptr,
+                       n->sqerr.ptr);
+  else
+    hipLaunchKernelGGL(mse_grad_kernel<PatchNet>, grid_of(B, 128), dim3(128), 0, st, B, n->O.ptr, T_dev, n->dO.ptr,
+                       n->sqerr.ptr);
   DOHP_CALL(backward_and_grad(n, B, X_dev, st));
   std::vector<float> e(B);
   DOHP_HIP(hipMemcpyAsync(e.data(), n->sqerr.ptr, sizeof(float) * B, hipMemcpyDeviceToHost, st));
-  if (grad_host) DOHP_HIP(hipMemcpyAsync(grad_host, n->grad.ptr, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, st));
+  if (grad_host)
+    DOHP_HIP(hipMemcpyAsync(grad_host, n->grad.ptr, sizeof(float) * nparam_of(n->kind), hipMemcpyDeviceToHost, st));
   DOHP_HIP(hipStreamSynchronize(st));
   if (loss) {
     double t = 0;
     for (int s = 0; s < B; s++) t += e[s];
-    *loss = t / ((double)B * NOUT);
+    *loss = t / ((double)B * nout_of(n->kind));
   }
   return 0;
 }
@@ -527,6 +606,7 @@ extern "C" dErr dohpMLPWeightStep(dohpMLP n, dohpMat q1, const float *X_dev, con
   PBox b;
   DOHP_CALL(box_of(q1, &b));
   DOHP_CHECK(n && X_dev && gs, dohpERR_ARG, "null argument");
+  DOHP_CHECK(n->kind == 0, dohpERR_ARG, "node-weight step needs a patch network (kind 0)");
   hipStream_t st = (hipStream_t)stream;
   const long nn = (long)b.nx * b.ny * b.nz, np = b.npatch();
   DOHP_CHECK(np > 0, dohpERR_ARG, "no patches");
@@ -535,10 +615,7 @@ extern "C" dErr dohpMLPWeightStep(dohpMLP n, dohpMat q1, const float *X_dev, con
   DOHP_CALL(dohpMLPWeights(n, q1, X_dev, s.ptr, nullptr, st));
   hipLaunchKernelGGL(dweights_kernel, grid_of(np), dim3(256), 0, st, b, n->O.ptr, s.ptr, gs, n->dO.ptr);
   DOHP_CALL(backward_and_grad(n, np, X_dev, st));
-  n->step++;
-  hipLaunchKernelGGL(adam_kernel, grid_of(NPARAM), dim3(256), 0, st, n->step, lr, 0.9f, 0.999f, 1e-8f, n->grad.ptr,
-                     n->master.ptr, n->m.ptr, n->v.ptr, n->wts.ptr);
-  DOHP_HIP(hipGetLastError());
+  DOHP_CALL(adam_step(n, lr, st));
   DOHP_HIP(hipStreamSynchronize(st));
   return 0;
 }
@@ -549,6 +626,7 @@ extern "C" dErr dohpMLPEnergyStep(dohpMLP n, dohpEllip op, dohpMat q1, dohpMG mg
   PBox b;
   DOHP_CALL(box_of(q1, &b));
   DOHP_CHECK(n && op && mg && X_dev && r && e, dohpERR_ARG, "null argument");
+  DOHP_CHECK(n->kind == 0, dohpERR_ARG, "node-weight step needs a patch network (kind 0)");
   hipStream_t st = (hipStream_t)stream;
   const long nn = (long)b.nx * b.ny * b.nz, np = b.npatch();
   DOHP_CHECK(np > 0, dohpERR_ARG, "no patches");
@@ -559,10 +637,33 @@ extern "C" dErr dohpMLPEnergyStep(dohpMLP n, dohpEllip op, dohpMat q1, dohpMG mg
   DOHP_CALL(energy_and_grad(op, mg, nn, s.ptr, r, e, alpha, gs.ptr, energy, nullptr, st));
   hipLaunchKernelGGL(dweights_kernel, grid_of(np), dim3(256), 0, st, b, n->O.ptr, s.ptr, gs.ptr, n->dO.ptr);
   DOHP_CALL(backward_and_grad(n, np, X_dev, st));
-  n->step++;
-  hipLaunchKernelGGL(adam_kernel, grid_of(NPARAM), dim3(256), 0, st, n->step, lr, 0.9f, 0.999f, 1e-8f, n->grad.ptr,
-                     n->master.ptr, n->m.ptr, n->v.ptr, n->wts.ptr);
+  DOHP_CALL(adam_step(n, lr, st));
+  DOHP_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+extern "C" dErr dohpMLPCellScales(dohpMLP n, long ncell, const float *X_dev, float *scales_dev, void *stream)
+{
+  DOHP_CHECK(n && X_dev && scales_dev && ncell > 0, dohpERR_ARG, "bad argument");
+  DOHP_CHECK(n->kind == 1, dohpERR_ARG, "sub-cell scales need a cell network (kind 1)");
+  hipStream_t st = (hipStream_t)stream;
+  DOHP_CALL(ensure(n, ncell));
+  DOHP_CALL(forward(n, ncell, X_dev, st));
+  hipLaunchKernelGGL(cell_scales_kernel, grid_of(3 * ncell), dim3(256), 0, st, ncell, n->O.ptr, scales_dev);
   DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" dErr dohpMLPCellStep(dohpMLP n, long ncell, const float *X_dev, const float *gsig_dev, float lr, void *stream)
+{
+  DOHP_CHECK(n && X_dev && gsig_dev && ncell > 0, dohpERR_ARG, "bad argument");
+  DOHP_CHECK(n->kind == 1, dohpERR_ARG, "sub-cell step needs a cell network (kind 1)");
+  hipStream_t st = (hipStream_t)stream;
+  DOHP_CALL(ensure(n, ncell));
+  DOHP_CALL(forward(n, ncell, X_dev, st));
+  hipLaunchKernelGGL(dcell_scales_kernel, grid_of(3 * ncell), dim3(256), 0, st, ncell, n->O.ptr, gsig_dev, n->dO.ptr);
+  DOHP_CALL(backward_and_grad(n, ncell, X_dev, st));
+  DOHP_CALL(adam_step(n, lr, st));
   DOHP_HIP(hipStreamSynchronize(st));
   return 0;
 }

@@ -12,9 +12,12 @@
 // dohpEllipSetQ1Mode(op, 1) assembles the corrected matrix instead (the Q1 finite-element stiffness of
 // the GLL sub-mesh): sub-cell derivatives +-1/h and the inverse Jacobian at each patch point.
 // The 8x8 sub-cell matrices are added into a CSR matrix over owned nodes (Dirichlet rows and
-// columns are dropped, src/fs/interface/fsbuild.c:31-34).  One thread per sub-cell.
+// columns are dropped, src/fs/interface/fsbuild.c:31-34).  One thread per sub-cell, in eight launches by the parity
+// of the sub-cell's lattice position: sub-cells of one colour share no node, so the blocks are added without atomics
+// in a fixed order and the matrix is bitwise reproducible.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -39,6 +42,15 @@ struct Q1Phys {
   double p, eps2, lambda;
   int has_k;
   int M1, M2, Ly, Lz;  // block extents: elements, and node lattice (the state is in lattice order)
+};
+
+// One colour of the sub-cell lattice: sub-cells whose global lattice position (e_d p + a_d) has parity c_d in every
+// direction d share no node, so a launch over one colour adds its blocks without atomics; the eight colours in
+// turn give a fixed summation order (the assembled matrix is bitwise reproducible).
+struct CellColor {
+  int c[3];    // parity per direction
+  int cnt[3];  // sub-cells of this colour per direction
+  __host__ __device__ long n() const { return (long)cnt[0] * cnt[1] * cnt[2]; }
 };
 
 __device__ inline void jac_at(const double (&X)[8][3], double x0, double x1, double x2, double (&J)[3][3])
@@ -110,20 +122,26 @@ __global__ void __launch_bounds__(128) q1_cell_kernel(int nelem, const int *__re
                                                      long nowned, const int *__restrict__ rowptr,
                                                      const int *__restrict__ colidx, double *__restrict__ val,
                                                      double *__restrict__ st, int corrected,
-                                                     const double *__restrict__ tab, Q1Phys ph)
+                                                     const double *__restrict__ tab, Q1Phys ph,
+                                                     const float *__restrict__ cscale, CellColor col)
 {
   constexpr int P = N - 1, NP3 = P * P * P, NQ = 2 * P;
   // per thread: u and the reference gradient at the 8 points
   __shared__ double PD[32][128];
   const double *zq = tab, *wq = tab + NQ, *hsub = tab + 2 * NQ, *Bs = tab + 2 * NQ + P, *Ds = Bs + NQ * N;
   const int lt = threadIdx.x;
-  const long tid = (long)blockIdx.x * blockDim.x + lt;
-  if (tid >= (long)nelem * NP3) return;
-  const long e = tid / NP3;
-  const int pidx = (int)(tid - e * NP3);
-  const int pa = pidx / (P * P), pb = (pidx / P) % P, pc = pidx % P;
+  const long t = (long)blockIdx.x * blockDim.x + lt;
+  if (t >= col.n()) return;
+  // sub-cell of this colour: global lattice position G_d = c_d + 2 t_d, element e_d = G_d / p, sub-cell G_d % p
+  const int t2 = (int)(t % col.cnt[2]), t1 = (int)((t / col.cnt[2]) % col.cnt[1]), t0 = (int)(t / ((long)col.cnt[1] * col.cnt[2]));
+  const int G0 = col.c[0] + 2 * t0, G1 = col.c[1] + 2 * t1, G2 = col.c[2] + 2 * t2;
+  const long e0 = G0 / P, e1 = G1 / P, e2 = G2 / P;
+  const int pa = G0 - (int)e0 * P, pb = G1 - (int)e1 * P, pc = G2 - (int)e2 * P;
+  const long e = (e0 * ph.M1 + e1) * ph.M2 + e2;
+  const int pidx = (pa * P + pb) * P + pc;
+  const long tid = e * NP3 + pidx;  // the sub-cell's index (learned scales)
+  (void)nelem;
   const int *ev = everts + e * 8;
-  const long e2 = e % ph.M2, e1 = (e / ph.M2) % ph.M1, e0 = e / ((long)ph.M1 * ph.M2);
   const long lat0 = ((e0 * P) * ph.Ly + e1 * P) * ph.Lz + e2 * P;
   {
     const double *bz0 = Bs + (2 * pc) * N, *bz1 = bz0 + N, *dz0 = Ds + (2 * pc) * N, *dz1 = dz0 + N;
@@ -182,8 +200,15 @@ __global__ void __launch_bounds__(128) q1_cell_kernel(int nelem, const int *__re
   }
   const double i1[2][2] = {{0.78867513459481287, 0.21132486540518713}, {0.21132486540518713, 0.78867513459481287}};
   const double d1[2][2] = {{-1., 1.}, {-1., 1.}};
-  const double sx = corrected ? 1.0 / hsub[pa] : 1.0, sy = corrected ? 1.0 / hsub[pb] : 1.0,
-               sz = corrected ? 1.0 / hsub[pc] : 1.0;
+  // sub-cell derivative scale per reference direction: 1 (the reference's), 1/h (corrected), times the learned
+  // per-sub-cell factors when given
+  double sx = corrected ? 1.0 / hsub[pa] : 1.0, sy = corrected ? 1.0 / hsub[pb] : 1.0,
+         sz = corrected ? 1.0 / hsub[pc] : 1.0;
+  if (cscale) {
+    sx *= (double)cscale[3 * tid];
+    sy *= (double)cscale[3 * tid + 1];
+    sz *= (double)cscale[3 * tid + 2];
+  }
   // the block is symmetric (EllipPointwiseJacobian below gives K[r][c] = -w lambda e^u I_r I_c + eta w D_r.D_c
   // - w (s.D_r)(s.D_c)): its upper triangle, row-major, is accumulated
   double K[36];
@@ -269,7 +294,7 @@ __global__ void __launch_bounds__(128) q1_cell_kernel(int nelem, const int *__re
         const int gc = gidx[c];
         if (gc >= nowned) continue;
         const int sl = ((c >> 2) - (r >> 2) + 1) * 9 + (((c >> 1) & 1) - ((r >> 1) & 1) + 1) * 3 + ((c & 1) - (r & 1) + 1);
-        atomicAdd(&st[sl * nowned + gr], K[sym_index(r, c)]);
+        st[sl * nowned + gr] += K[sym_index(r, c)];  // (no other sub-cell of this colour holds the entry)
       }
     } else {
       const int lo = rowptr[gr], hi = rowptr[gr + 1];
@@ -277,7 +302,7 @@ __global__ void __launch_bounds__(128) q1_cell_kernel(int nelem, const int *__re
       for (int c = 0; c < 8; c++) {
         const int gc = gidx[c];
         if (gc >= nowned) continue;
-        atomicAdd(&val[find_col(colidx, lo, hi, gc)], K[sym_index(r, c)]);
+        val[find_col(colidx, lo, hi, gc)] += K[sym_index(r, c)];
       }
     }
   }
@@ -323,11 +348,163 @@ __global__ void zero_f64_kernel(double *__restrict__ a, long n)
   if (i < n) a[i] = 0.0;
 }
 
+// GLL node a of the element from the sub-interval widths.
+__device__ inline double gll_node(const double *hsub, int a)
+{
+  double z = -1.0;
+  for (int i = 0; i < a; i++) z += hsub[i];
+  return z;
+}
+
+// Features of each GLL sub-cell for the learned derivative scales (kQ1CellInputs values): the log widths of its
+// three reference sub-intervals, its centre in element-reference coordinates, the log physical lengths of the three
+// reference directions at the centre minus their mean, log det J minus their sum (skew, <= 0), and log k at the
+// centre minus the mean log k of the element's vertices (0 without a coefficient).
+template <int N>
+__global__ void q1_cell_features_kernel(int nelem, const int *__restrict__ everts, const double *__restrict__ vxyz,
+                                        const double *__restrict__ kv, const double *__restrict__ tab,
+                                        float *__restrict__ X)
+{
+  constexpr int P = N - 1, NP3 = P * P * P, NQ = 2 * P;
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long)nelem * NP3) return;
+  const long e = tid / NP3;
+  const int pidx = (int)(tid - e * NP3), pa = pidx / (P * P), pb = (pidx / P) % P, pc = pidx % P;
+  const double *hsub = tab + 2 * NQ;
+  const int *ev = everts + e * 8;
+  double Xv[8][3];
+  for (int t = 0; t < 8; t++)
+    for (int d = 0; d < 3; d++) Xv[t][d] = vxyz[3 * (long)ev[t] + d];
+  const double c0 = gll_node(hsub, pa) + 0.5 * hsub[pa], c1 = gll_node(hsub, pb) + 0.5 * hsub[pb],
+               c2 = gll_node(hsub, pc) + 0.5 * hsub[pc];
+  double J[3][3], Ji[3][3];
+  jac_at(Xv, c0, c1, c2, J);
+  const double det = invert3(J, Ji);
+  double len[3], mean = 0;
+  for (int a = 0; a < 3; a++) {
+    len[a] = 0.5 * log(J[0][a] * J[0][a] + J[1][a] * J[1][a] + J[2][a] * J[2][a]);
+    mean += len[a] / 3;
+  }
+  double lk = 0;
+  if (kv) {
+    const double f[3][2] = {{0.5 * (1 - c0), 0.5 * (1 + c0)}, {0.5 * (1 - c1), 0.5 * (1 + c1)},
+                            {0.5 * (1 - c2), 0.5 * (1 + c2)}};
+    double kc = 0, lmean = 0;
+    for (int t = 0; t < 8; t++) {
+      kc += kv[ev[t]] * f[0][t >> 2] * f[1][(t >> 1) & 1] * f[2][t & 1];
+      lmean += log(kv[ev[t]]) / 8;
+    }
+    lk = log(kc) - lmean;
+  }
+  float *x = X + tid * kQ1CellInputs;
+  x[0] = (float)log(hsub[pa]);
+  x[1] = (float)log(hsub[pb]);
+  x[2] = (float)log(hsub[pc]);
+  x[3] = (float)c0;
+  x[4] = (float)c1;
+  x[5] = (float)c2;
+  for (int a = 0; a < 3; a++) x[6 + a] = (float)(len[a] - mean);
+  x[9] = (float)(log(det) - 3 * mean);
+  x[10] = (float)lk;
+}
+
+// Gradient of the bilinear form w^T K(sigma) z of the linear operator's embedded-Q1 matrix (ellip p = 2, lambda = 0)
+// in the sub-cell derivative scales: the sub-cell block is K = sum_q W_q D_q D_q^T with D_q = sum_a sigma_a U_qa
+// (U_qa: the +-1 sub-cell derivative along reference direction a times the inverse Jacobian row a, as assembled), so
+// d(w^T K z)/d sigma_a = sum_q W_q [(U_qa^T w) . (D_q^T z) + (D_q^T w) . (U_qa^T z)].  g[cell][a] += coef times it.
+// w and z are vectors over the owned nodes (zero on Dirichlet nodes, whose rows and columns are dropped).
+template <int N>
+__global__ void __launch_bounds__(128) q1_scale_grad_kernel(int nelem, const int *__restrict__ enodes,
+                                                           const int *__restrict__ everts,
+                                                           const double *__restrict__ vxyz,
+                                                           const double *__restrict__ kv, long nowned, int corrected,
+                                                           const double *__restrict__ tab,
+                                                           const float *__restrict__ cscale,
+                                                           const double *__restrict__ wv, const double *__restrict__ zv,
+                                                           double coef, float *__restrict__ g)
+{
+  constexpr int P = N - 1, NP3 = P * P * P, NQ = 2 * P;
+  const double *zq = tab, *wq = tab + NQ, *hsub = tab + 2 * NQ;
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long)nelem * NP3) return;
+  const long e = tid / NP3;
+  const int pidx = (int)(tid - e * NP3), pa = pidx / (P * P), pb = (pidx / P) % P, pc = pidx % P;
+  const int *ev = everts + e * 8;
+  const int *en = &enodes[e * N * N * N];
+  double w8[8], z8[8];
+  for (int b = 0; b < 8; b++) {
+    const int gi = en[((pa + (b >> 2)) * N + pb + ((b >> 1) & 1)) * N + pc + (b & 1)];
+    w8[b] = gi < nowned ? wv[gi] : 0.0;
+    z8[b] = gi < nowned ? zv[gi] : 0.0;
+  }
+  const double base[3] = {corrected ? 1.0 / hsub[pa] : 1.0, corrected ? 1.0 / hsub[pb] : 1.0,
+                          corrected ? 1.0 / hsub[pc] : 1.0};
+  double sig[3] = {1.0, 1.0, 1.0};
+  if (cscale)
+    for (int a = 0; a < 3; a++) sig[a] = (double)cscale[3 * tid + a];
+  double Xv[8][3];
+  for (int t = 0; t < 8; t++)
+    for (int d = 0; d < 3; d++) Xv[t][d] = vxyz[3 * (long)ev[t] + d];
+  const double i1[2][2] = {{0.78867513459481287, 0.21132486540518713}, {0.21132486540518713, 0.78867513459481287}};
+  const double d1[2][2] = {{-1., 1.}, {-1., 1.}};
+  double acc[3] = {0, 0, 0};
+  for (int q = 0; q < 8; q++) {
+    const int ii = q >> 2, jj = (q >> 1) & 1, kk = q & 1;
+    const int qi = 2 * pa + ii, qj = 2 * pb + jj, qk = 2 * pc + kk;
+    double J[3][3], Ji[3][3], cj[3][3];
+    jac_at(Xv, zq[qi], zq[qj], zq[qk], J);
+    const double det = invert3(J, Ji);
+    double kq = 1.0;
+    if (kv) {
+      const double f0[2] = {0.5 * (1 - zq[qi]), 0.5 * (1 + zq[qi])};
+      const double f1[2] = {0.5 * (1 - zq[qj]), 0.5 * (1 + zq[qj])};
+      const double f2[2] = {0.5 * (1 - zq[qk]), 0.5 * (1 + zq[qk])};
+      kq = 0;
+      for (int t = 0; t < 8; t++) kq += kv[ev[t]] * f0[t >> 2] * f1[(t >> 1) & 1] * f2[t & 1];
+    }
+    if (corrected) {
+      for (int r = 0; r < 3; r++)
+        for (int d = 0; d < 3; d++) cj[r][d] = Ji[r][d];
+    } else {  // element-natural point pidx*8+q, as the assembly
+      const int qe = pidx * 8 + q;
+      const int ei = qe / (NQ * NQ), ej = (qe / NQ) % NQ, ek = qe % NQ;
+      double J2[3][3];
+      jac_at(Xv, zq[ei], zq[ej], zq[ek], J2);
+      invert3(J2, cj);
+    }
+    const double W = wq[qi] * wq[qj] * wq[qk] * det * kq;
+    // reference derivative sums along each direction: sum_b dd_a[b] v_b
+    double rw[3] = {0, 0, 0}, rz[3] = {0, 0, 0};
+    for (int b = 0; b < 8; b++) {
+      const int a0 = b >> 2, a1 = (b >> 1) & 1, a2 = b & 1;
+      const double dd[3] = {d1[ii][a0] * i1[jj][a1] * i1[kk][a2], i1[ii][a0] * d1[jj][a1] * i1[kk][a2],
+                            i1[ii][a0] * i1[jj][a1] * d1[kk][a2]};
+      for (int a = 0; a < 3; a++) {
+        rw[a] += dd[a] * w8[b];
+        rz[a] += dd[a] * z8[b];
+      }
+    }
+    double Uw[3][3], Uz[3][3], Dw[3] = {0, 0, 0}, Dz[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++)
+      for (int d = 0; d < 3; d++) {
+        Uw[a][d] = base[a] * rw[a] * cj[a][d];
+        Uz[a][d] = base[a] * rz[a] * cj[a][d];
+        Dw[d] += sig[a] * Uw[a][d];
+        Dz[d] += sig[a] * Uz[a][d];
+      }
+    for (int a = 0; a < 3; a++)
+      acc[a] += W * (Uw[a][0] * Dz[0] + Uw[a][1] * Dz[1] + Uw[a][2] * Dz[2] + Dw[0] * Uz[a][0] + Dw[1] * Uz[a][1] +
+                     Dw[2] * Uz[a][2]);
+  }
+  for (int a = 0; a < 3; a++) g[3 * tid + a] += (float)(coef * acc[a]);
+}
+
 }  // namespace
 
 // Accessors the operator file uses (defined there).
 extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpEllipParam *prm, const double **kvert,
                                               const double **state, int *q1mode);
+const float *dohpEllipGetQ1CellScales_Private(dohpEllip op);
 
 // CSR pattern of the 27-point stencil on a box of rows in lexicographic order, built on the device: row (i, j, k)
 // couples to its box neighbours in increasing index, and its offset has a closed form (the count of entries of the
@@ -436,6 +613,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   int q1mode = 0;
   DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kvert, &state, &q1mode));
   DOHP_CHECK(fs->p <= kMaxP, dohpERR_ARG, "Q1 assembly supports degree <= 10");
+  const float *cscale = dohpEllipGetQ1CellScales_Private(op);
   hipStream_t st = (hipStream_t)stream;
   if (u) DOHP_CALL(dohpEllipFunctionState_Private(op, u, st));
   _p_dohpMat *A = *out;
@@ -473,7 +651,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
             fs->p * fs->M[1] + 1, fs->p * fs->M[2] + 1};
   const long nt = (long)fs->nelem * fs->p * fs->p * fs->p;
   // On a block space the owned nodes form a box and the matrix is a 27-point stencil: the sub-cells add into the
-  // stencil form directly (coalesced atomics at computed slots), and the CSR values are read back from it.
+  // stencil form directly (at computed slots, colour by colour), and the CSR values are read back from it.
   bool box = false;
   DOHP_CALL(dohpMatPrepareStencil_Private(A, fs, &box, st));
   if (box) {
@@ -494,14 +672,23 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
     DOHP_HIP(hipMemcpyAsync(A->q1tab.ptr, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, st));
     DOHP_HIP(hipStreamSynchronize(st));  // the host vector goes out of scope
   }
-  const dim3 grid((unsigned)((nt + 127) / 128));
+  (void)nt;
   double *stp = box ? A->st64.ptr : nullptr;
+  for (int color = 0; color < 8; color++) {
+  CellColor cc;
+  for (int d = 0; d < 3; d++) {
+    cc.c[d] = (color >> (2 - d)) & 1;
+    const int G = fs->M[d] * fs->p;  // sub-cells per direction
+    cc.cnt[d] = G > cc.c[d] ? (G - cc.c[d] + 1) / 2 : 0;
+  }
+  if (cc.n() == 0) continue;
+  const dim3 grid((unsigned)((cc.n() + 127) / 128));
   switch (fs->n) {
 #define DOHP_CASE(NN)                                                                                              \
   case NN:                                                                                                         \
     hipLaunchKernelGGL(q1_cell_kernel<NN>, grid, dim3(128), 0, st, fs->nelem, fs->d_elem_nodes.ptr,               \
                        fs->d_elem_verts.ptr, fs->d_vcoords.ptr, kvert, state, fs->nowned, A->rowptr.ptr, A->col.ptr, \
-                       A->val.ptr, stp, tb.corrected, A->q1tab.ptr, ph);                                          \
+                       A->val.ptr, stp, tb.corrected, A->q1tab.ptr, ph, cscale, cc);                              \
     break;
     DOHP_CASE(2)
     DOHP_CASE(3)
@@ -517,6 +704,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   default: set_error("embedded-Q1 assembly supports degree 1..10"); return dohpERR_ARG;
   }
   DOHP_HIP(hipGetLastError());
+  }
   if (box) return dohpMatStencilToCSR_Private(A, st);
   hipLaunchKernelGGL(to_f32_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->val32.ptr, A->nnz);
   DOHP_HIP(hipGetLastError());
@@ -581,4 +769,80 @@ extern "C" dErr dohpMatDestroy(dohpMat *A)
     *A = nullptr;
   }
   return 0;
+}
+
+// ------------------------------------------------------ learned sub-cell scales of the embedded-Q1 matrix
+
+namespace {
+
+template <typename F>
+dErr q1_dispatch(int n, F f)
+{
+  switch (n) {
+  case 2: f(std::integral_constant<int, 2>()); break;
+  case 3: f(std::integral_constant<int, 3>()); break;
+  case 4: f(std::integral_constant<int, 4>()); break;
+  case 5: f(std::integral_constant<int, 5>()); break;
+  case 6: f(std::integral_constant<int, 6>()); break;
+  case 7: f(std::integral_constant<int, 7>()); break;
+  case 8: f(std::integral_constant<int, 8>()); break;
+  case 9: f(std::integral_constant<int, 9>()); break;
+  case 10: f(std::integral_constant<int, 10>()); break;
+  case 11: f(std::integral_constant<int, 11>()); break;
+  default: set_error("embedded-Q1 assembly supports degree 1..10"); return dohpERR_ARG;
+  }
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" dErr dohpQ1CellCount(dohpEllip op, long *ncell)
+{
+  DOHP_CHECK(op && ncell, dohpERR_ARG, "null argument");
+  dohpFS fs;
+  dohpEllipParam prm;
+  const double *kv, *stt;
+  DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kv, &stt, nullptr));
+  *ncell = (long)fs->nelem * fs->p * fs->p * fs->p;
+  return 0;
+}
+
+extern "C" dErr dohpQ1CellFeatures(dohpEllip op, dohpMat q1, float *X, void *stream)
+{
+  DOHP_CHECK(op && q1 && X, dohpERR_ARG, "null argument");
+  DOHP_CHECK(q1->q1tab.ptr, dohpERR_ARG, "the embedded-Q1 matrix has not been assembled");
+  dohpFS fs;
+  dohpEllipParam prm;
+  const double *kv, *stt;
+  DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kv, &stt, nullptr));
+  const long nt = (long)fs->nelem * fs->p * fs->p * fs->p;
+  const dim3 grid((unsigned)((nt + 127) / 128));
+  hipStream_t st = (hipStream_t)stream;
+  return q1_dispatch(fs->n, [&](auto NN) {
+    hipLaunchKernelGGL(q1_cell_features_kernel<decltype(NN)::value>, grid, dim3(128), 0, st, fs->nelem,
+                       fs->d_elem_verts.ptr, fs->d_vcoords.ptr, kv, q1->q1tab.ptr, X);
+  });
+}
+
+extern "C" dErr dohpQ1CellScaleGradient(dohpEllip op, dohpMat q1, const double *w, const double *z, double coef,
+                                        float *g, void *stream)
+{
+  DOHP_CHECK(op && q1 && w && z && g, dohpERR_ARG, "null argument");
+  DOHP_CHECK(q1->q1tab.ptr, dohpERR_ARG, "the embedded-Q1 matrix has not been assembled");
+  dohpFS fs;
+  dohpEllipParam prm;
+  const double *kv, *stt;
+  DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kv, &stt, nullptr));
+  DOHP_CHECK(prm.p == 2.0 && prm.lambda == 0.0, dohpERR_ARG,
+             "the sub-cell scale gradient is for the linear operator (ellip p = 2, lambda = 0)");
+  const float *cscale = dohpEllipGetQ1CellScales_Private(op);
+  const long nt = (long)fs->nelem * fs->p * fs->p * fs->p;
+  const dim3 grid((unsigned)((nt + 127) / 128));
+  hipStream_t st = (hipStream_t)stream;
+  return q1_dispatch(fs->n, [&](auto NN) {
+    hipLaunchKernelGGL(q1_scale_grad_kernel<decltype(NN)::value>, grid, dim3(128), 0, st, fs->nelem,
+                       fs->d_elem_nodes.ptr, fs->d_elem_verts.ptr, fs->d_vcoords.ptr, kv, fs->nowned, q1->corrected,
+                       q1->q1tab.ptr, cscale, w, z, coef, g);
+  });
 }

@@ -191,6 +191,7 @@ template <typename T>
 struct Prec {
   Cheb<T> *cheb = nullptr;
   dohpMG mg = nullptr;
+  dohpMG mg_fb = nullptr;  // plain multigrid taking over from mg (a learned embedded-Q1 matrix's) when learned_on drops
   const double *s = nullptr;  // learned node weights, or null
   T *tmp = nullptr;
   bool learned_on = false;
@@ -217,6 +218,7 @@ dErr Prec<T>::apply(Ctx &c, const T *r, T *z)
     }
   }
   if (!mg) return cheb->apply(c, r, z);
+  if (mg_fb && !learned_on) return mg_apply<T>(mg_fb, r, z, c.st);
   if (!(s && learned_on)) return mg_apply<T>(mg, r, z, c.st);
   const int f32 = sizeof(T) == 4;
   hipLaunchKernelGGL(scale2_kernel, grid(c.n), dim3(256), 0, c.st, c.n, s, (const void *)r, (void *)tmp, f32);
@@ -339,12 +341,27 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double
 
 namespace {
 
+// Multigrid hierarchy of q1 at smoothing degree `deg` (0 = the default for its arithmetic), kept with the matrix.
+dErr hierarchy(dohpMat q1, int deg, hipStream_t st, dohpMG *mg)
+{
+  deg = deg > 0 ? deg : (q1->corrected ? 1 : 2);
+  if (!q1->mg || q1->mg_degree != deg) {
+    dohpMGDestroy(&q1->mg);
+    DOHP_CALL(dohpMGCreate(q1, deg, 0, &q1->mg, st));
+    q1->mg_degree = deg;
+  }
+  *mg = q1->mg;
+  return 0;
+}
+
 dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *x, const dohpSolveOpts *opts,
-                dohpSolveStats *stats, double *snap, int nsnap, int every, int *nsnapped, hipStream_t st)
+                dohpSolveStats *stats, double *snap, int nsnap, int every, int *nsnapped, hipStream_t st,
+                dohpMat q1fb = nullptr)
 {
   DOHP_CHECK(op && q1 && b && x && opts && stats, dohpERR_ARG, "null argument");
   DOHP_CHECK(opts->pc >= 0 && opts->pc <= 1, dohpERR_ARG, "pc must be 0 (Jacobi-Chebyshev) or 1 (multigrid)");
   DOHP_CHECK(!mlp || opts->pc == 1, dohpERR_ARG, "the learned correction applies to the multigrid preconditioner");
+  DOHP_CHECK(!q1fb || opts->pc == 1, dohpERR_ARG, "the fallback matrix applies to the multigrid preconditioner");
   const auto t0 = std::chrono::steady_clock::now();
   Ctx c;
   c.st = st;
@@ -372,21 +389,16 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
   stats->lmax = lmax;
   const int degree = opts->cheb_degree > 0 ? opts->cheb_degree : 4;
   struct {
-    dohpMG mg = nullptr;
+    dohpMG mg = nullptr, mg_fb = nullptr;
   } mgh;
   if (opts->pc == 1) {  // the hierarchy is kept with the matrix until it is refilled
     // default smoothing degree: 1 on the corrected Q1 matrix (fastest, measured), 2 on the reference arithmetic's
     // (where degree 1 with the FP32 V-cycle can stall)
-    const int deg = opts->mg_degree > 0 ? opts->mg_degree : (q1->corrected ? 1 : 2);
-    if (!q1->mg || q1->mg_degree != deg) {
-      dohpMGDestroy(&q1->mg);
-      DOHP_CALL(dohpMGCreate(q1, deg, 0, &q1->mg, c.st));
-      q1->mg_degree = deg;
-    }
-    mgh.mg = q1->mg;
+    DOHP_CALL(hierarchy(q1, opts->mg_degree, c.st, &mgh.mg));
+    if (q1fb) DOHP_CALL(hierarchy(q1fb, opts->mg_degree, c.st, &mgh.mg_fb));
   }
   Budget bud;
-  bud.fallback_its = mlp ? opts->fallback_its : 0;
+  bud.fallback_its = mlp || q1fb ? opts->fallback_its : 0;
   // learned node weights (computed once: the features do not depend on the residual)
   DevBuf<double> wts;
   DevBuf<double> tmp64;
@@ -417,9 +429,10 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     Prec<float> M32;
     M32.cheb = &C32;
     M32.mg = mgh.mg;
+    M32.mg_fb = mgh.mg_fb;
     M32.s = wts.ptr;
     M32.tmp = tmp32.ptr;
-    M32.learned_on = mlp != nullptr;
+    M32.learned_on = mlp != nullptr || q1fb != nullptr;
     DevBuf<float> lr, lz;
     DOHP_CALL(lr.alloc(c.n));
     DOHP_CALL(lz.alloc(c.n));
@@ -442,9 +455,10 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     Prec<double> M;
     M.cheb = &C64;
     M.mg = mgh.mg;
+    M.mg_fb = mgh.mg_fb;
     M.s = wts.ptr;
     M.tmp = tmp64.ptr;
-    M.learned_on = mlp != nullptr;
+    M.learned_on = mlp != nullptr || q1fb != nullptr;
     int its;
     double rn, bn;
     DOHP_CALL(pcg<double>(c, op, M, bud, b, x, opts->rtol, opts->max_it, &its, &rn, &bn, snap, nsnap, every, nsnapped));
@@ -460,9 +474,10 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     Prec<float> M;
     M.cheb = &C32;
     M.mg = mgh.mg;
+    M.mg_fb = mgh.mg_fb;
     M.s = wts.ptr;
     M.tmp = tmp32.ptr;
-    M.learned_on = mlp != nullptr;
+    M.learned_on = mlp != nullptr || q1fb != nullptr;
     DevBuf<double> r, t;
     DevBuf<float> r32, e32;
     DOHP_CALL(r.alloc(c.n));
@@ -528,4 +543,11 @@ extern "C" dErr dohpEllipSolveRecord(dohpEllip op, dohpMat q1, const double *b, 
   DOHP_CHECK(opts && !opts->mixed, dohpERR_ARG, "iterates are recorded by the plain FP64 solve");
   DOHP_CHECK(snap_dev && nsnap > 0 && every > 0 && nsnapped, dohpERR_ARG, "bad snapshot arguments");
   return solve_impl(op, q1, nullptr, b, x, opts, stats, snap_dev, nsnap, every, nsnapped, (hipStream_t)stream);
+}
+
+extern "C" dErr dohpEllipSolveFallback(dohpEllip op, dohpMat q1, dohpMat q1_fallback, const double *b, double *x,
+                                       const dohpSolveOpts *opts, dohpSolveStats *stats, void *stream)
+{
+  DOHP_CHECK(q1_fallback, dohpERR_ARG, "null fallback matrix");
+  return solve_impl(op, q1, nullptr, b, x, opts, stats, nullptr, 0, 0, nullptr, (hipStream_t)stream, q1_fallback);
 }

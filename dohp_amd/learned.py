@@ -196,3 +196,152 @@ def fit(C, torch, train_seeds, val_seeds, M=(12, 12, 12), deg=6, epochs=30, lr=2
     report = {"train_problems": len(probs), "val_cg_its_plain": plain_its, "best_epoch": best[2],
               "val_cg_its_learned": best[0] if final else None, "history": hist, "useful": final is not None}
     return final, report
+
+
+# ----------------------------------------------------------------------------------------------------------------
+# Learned sub-cell scales of the embedded-Q1 matrix (network kind 1).
+#
+# The embedded-Q1 matrix in the reference's arithmetic takes sub-cell derivatives of +-1 per element-reference unit
+# whatever the width of the GLL sub-interval, which makes it a poor spectral match for the high-order operator away
+# from the element centres.  A 3-layer perceptron maps the features of each GLL sub-cell (its reference widths,
+# position, physical shape and the local coefficient) to one scale per reference direction multiplying those
+# derivatives.  It is trained on residual/error pairs (r, e) of plain solves: the preconditioner z = M(sigma) r (the
+# multigrid V-cycle on the matrix assembled with the scales) should reproduce e in the energy norm of the high-order
+# operator A, up to one scale alpha shared by the pairs:
+#     E = mean_pairs || e - alpha z ||_A^2 / || e ||_A^2.
+# With dM = -M dK M (M an approximate inverse of the Q1 matrix K), dE/dsigma = mean 2 alpha w^T (dK/dsigma) z /
+# ||e||_A^2 with w = M A (e - alpha z); dohpQ1CellScaleGradient evaluates the bilinear form per sub-cell, the network's
+# HIP kernels backpropagate it and take an Adam step.  The solve then preconditions with the learned matrix; the plain
+# matrix takes over after as many iterations as the plain solve needed (fallback).
+
+def cell_init_params(nparam, seed=0):
+    """Xavier-uniform hidden layers, zero output layer: every scale 1 (the plain matrix)."""
+    nin, nh, nout = 11, 64, 3
+    rng = np.random.default_rng(seed)
+    w1 = rng.uniform(-1, 1, (nh, nin)) * np.sqrt(6.0 / (nin + nh))
+    w2 = rng.uniform(-1, 1, (nh, nh)) * np.sqrt(6.0 / (2 * nh))
+    p = np.concatenate([w1.ravel(), np.zeros(nh), w2.ravel(), np.zeros(nh), np.zeros(nout * nh), np.zeros(nout)])
+    assert p.size == nparam
+    return p.astype(np.float32)
+
+
+class ScaledProblem:
+    """A log-normal Poisson problem with its plain embedded-Q1 matrix, sub-cell features, the learned matrix
+    (refilled per set of scales) and, for training, residual/error pairs of a plain solve."""
+
+    def __init__(self, C, torch, M, deg, seed, q1_mode=0, npairs=0, every=3, nrandom=0):
+        self.C, self.torch = C, torch
+        self.fs, self.op, self.q1, self.b, self.xs = make_problem(C, M, deg, seed, torch, q1_mode=q1_mode)
+        n = self.fs.nowned
+        self.ncell = self.op.q1_cell_count()
+        self.X = torch.zeros((self.ncell, C.CellMLP.ninputs), dtype=torch.float32, device="cuda")
+        self.op.q1_cell_features(self.q1, self.X)
+        self.sig = torch.ones((self.ncell, 3), dtype=torch.float32, device="cuda")
+        self.q1s = None
+        self.pairs = []
+        self.plain_its = None
+        if npairs:
+            snap = torch.zeros((npairs, n), dtype=torch.float64, device="cuda")
+            x = torch.zeros_like(self.b)
+            st = self.op.solve(self.q1, self.b, x, pc="mg", rtol=1e-11, record=(snap, every))
+            self.plain_its = st["inner_its"]
+            errs = [x - snap[k] for k in range(st["nsnap"])]
+            for k in range(nrandom):
+                errs.append(torch.as_tensor(2 * splitmix64_uniform(7919 * seed + k, n) - 1, dtype=torch.float64,
+                                            device="cuda"))
+            for e in errs:
+                r = torch.zeros_like(e)
+                self.op.mat_mult(e, r)  # r = A e (= b - A x_k for the iterates)
+                self.pairs.append((r, e))
+
+    def assemble(self, net):
+        """Learned matrix for the network's scales (net None: unit scales)."""
+        if net is None:
+            self.sig.fill_(1.0)
+        else:
+            net.scales(self.X, self.sig)
+        self.op.set_q1_cell_scales(self.sig)
+        self.q1s = self.op.assemble_q1(torch_zeros(self.torch, self.b), mat=self.q1s)
+        self.op.set_q1_cell_scales(None)
+        return self.q1s
+
+    def energy_gradient(self, gsig):
+        """Mean relative energy error of the learned V-cycle over the pairs (shared optimal scale) and its gradient
+        in the sub-cell scales added into gsig."""
+        torch, C = self.torch, self.C
+        mg = C.MG(self.q1s, smooth_degree=2)
+        zs, terms = [], []
+        for (r, e) in self.pairs:
+            z = torch.zeros_like(r)
+            mg.apply(r, z)
+            Az = torch.zeros_like(z)
+            self.op.mat_mult(z, Az)
+            eAe = float(torch.dot(e, r))
+            terms.append((float(torch.dot(z, r)) / eAe, float(torch.dot(z, Az)) / eAe, eAe))
+            zs.append(z)
+        alpha = sum(t[0] for t in terms) / sum(t[1] for t in terms)
+        Es = []
+        self.op.set_q1_cell_scales(self.sig)  # the gradient is taken at the scales of the assembled matrix
+        for (r, e), z, (_, _, eAe) in zip(self.pairs, zs, terms):
+            d = e - alpha * z
+            Ad = torch.zeros_like(d)
+            self.op.mat_mult(d, Ad)
+            Es.append(float(torch.dot(d, Ad)) / eAe)
+            w = torch.zeros_like(d)
+            mg.apply(Ad, w)
+            self.op.q1_cell_scale_gradient(self.q1s, w, z, 2.0 * alpha / (eAe * len(self.pairs)), gsig)
+        self.op.set_q1_cell_scales(None)
+        return float(np.mean(Es))
+
+    def cg_its(self, q1=None, fallback_its=0, **kw):
+        x = self.torch.zeros_like(self.b)
+        if q1 is None:
+            st = self.op.solve(self.q1, self.b, x, pc="mg", rtol=1e-10, **kw)
+        else:
+            st = self.op.solve(q1, self.b, x, pc="mg", rtol=1e-10, fallback_q1=self.q1, fallback_its=fallback_its, **kw)
+        return st["inner_its"]
+
+
+def torch_zeros(torch, like):
+    return torch.zeros_like(like)
+
+
+def fit_q1_scales(C, torch, train_seeds, val_seeds, M=(12, 12, 12), deg=6, epochs=30, lr=3e-3, seed=0, npairs=8,
+                  every=3, nrandom=2, val_M=None, scale_penalty=0.0):
+    """Train the sub-cell scale network on residual/error pairs of plain solves of the training problems (one Adam
+    step per problem and epoch on the mean energy error of its pairs); keep the parameters with the fewest CG
+    iterations on the validation problems (of size val_M, default M).  scale_penalty adds
+    0.5 * scale_penalty * (mean log sigma)^2: the loss does not see a common factor of all scales (alpha absorbs it),
+    and the penalty keeps that factor near 1 instead of drifting to the clamp.  Returns (net or None if no epoch beat
+    the plain matrix, report)."""
+    train = [ScaledProblem(C, torch, M, deg, sd, npairs=npairs, every=every, nrandom=nrandom) for sd in train_seeds]
+    val = [ScaledProblem(C, torch, val_M or M, deg, sd) for sd in val_seeds]
+    plain_its = [v.cg_its() for v in val]
+    net = C.CellMLP(cell_init_params(C.CellMLP.nparam, seed))
+    best = (sum(plain_its), None, -1)
+    hist = []
+    for ep in range(epochs):
+        Es = []
+        for p in train:
+            p.assemble(net)
+            gsig = torch.zeros((p.ncell, 3), dtype=torch.float32, device="cuda")
+            Es.append(p.energy_gradient(gsig))
+            if scale_penalty:
+                ls = torch.log(p.sig)
+                gsig += scale_penalty * float(ls.mean()) / ls.numel() / p.sig
+            net.step(p.X, gsig, lr)
+        its = []
+        for v in val:
+            its.append(v.cg_its(v.assemble(net)))
+        sig = val[0].sig
+        hist.append({"epoch": ep, "train_energy": float(np.mean(Es)), "val_cg_its": its,
+                     "sigma_range": [float(sig.min()), float(sig.max())]})
+        if sum(its) < best[0]:
+            best = (sum(its), net.params()[0].copy(), ep)
+    final = C.CellMLP(best[1]) if best[1] is not None else None
+    npair_total = sum(len(p.pairs) for p in train)
+    report = {"train_problems": len(train), "train_pairs": npair_total,
+              "train_pair_unknowns": int(sum(len(p.pairs) * p.fs.nowned for p in train)),
+              "val_cg_its_plain": plain_its, "best_epoch": best[2], "val_cg_its_learned": best[0] if final else None,
+              "history": hist, "useful": final is not None}
+    return final, report

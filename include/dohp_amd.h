@@ -231,6 +231,22 @@ dErr dohpEllipAssembleQ1(dohpEllip op, const double *u_dev, dohpMat *mat, void *
  * Jacobian index of fsrulesetit.c:684), 1 = corrected: derivatives +-1/h of the GLL sub-interval and the
  * inverse Jacobian at each point, i.e. the Q1 finite-element stiffness of the GLL sub-mesh. */
 dErr dohpEllipSetQ1Mode(dohpEllip op, int mode);
+/* Learned sub-cell derivative scales of later embedded-Q1 assemblies (no counterpart in the reference): the
+ * sub-cell derivatives along the element's three reference directions (+-1 in the reference, efstopo.c:665-666)
+ * are multiplied by scales_dev[cell][3], cell = element * p^3 + (a p + b) p + c for GLL sub-cell (a, b, c).
+ * The array is the caller's and is read by every later assembly until replaced; NULL = none. */
+dErr dohpEllipSetQ1CellScales(dohpEllip op, const float *scales_dev);
+/* Number of GLL sub-cells, nelem * p^3. */
+dErr dohpQ1CellCount(dohpEllip op, long *ncell);
+/* Features of every sub-cell for the learned scales, X [ncell][dohpMLPNumInputsKind(1)]: log widths of its three
+ * reference sub-intervals, its centre (element-reference coordinates), log physical lengths of the reference
+ * directions at the centre minus their mean, log det J minus their sum, log k at the centre minus the element's
+ * mean log k.  q1 must have been assembled by this operator (its 1-D tables are used). */
+dErr dohpQ1CellFeatures(dohpEllip op, dohpMat q1, float *X_dev, void *stream);
+/* g[cell][a] += coef * w^T (dK / dsigma_{cell,a}) z: gradient of the bilinear form of the linear operator's
+ * (ellip p = 2, lambda = 0) embedded-Q1 matrix in the sub-cell scales set on op; w, z over the owned nodes. */
+dErr dohpQ1CellScaleGradient(dohpEllip op, dohpMat q1, const double *w_dev, const double *z_dev, double coef,
+                             float *g_dev, void *stream);
 dErr dohpMatGetSizes(dohpMat A, long *nrows, long *nnz);
 /* Copies the CSR arrays to host buffers (rowptr: nrows+1, col/val: nnz); any may be NULL. */
 dErr dohpMatGetCSR(dohpMat A, int *rowptr_host, int *col_host, double *val_host);
@@ -278,6 +294,19 @@ int dohpMLPNumParams(void);
 int dohpMLPNumInputs(void);
 /* params_host NULL = all zeros (all weights 1: the plain preconditioner). */
 dErr dohpMLPCreate(const float *params_host, dohpMLP *mlp);
+/* Network kinds: 0 = the patch node weights above; 1 = the sub-cell scales of the embedded-Q1 matrix, the same
+ * 3-layer perceptron (11 -> 64 -> 64 -> 3, SiLU, bfloat16 weights, FP32 accumulation) mapping the features of a
+ * GLL sub-cell (dohpQ1CellFeatures) to sigma = exp(clamp(o, -4, 4)) per reference direction.  Zero parameters
+ * give sigma = 1: the plain (reference) matrix.  Sizes per kind (-1 for an unknown kind): */
+int dohpMLPNumParamsKind(int kind);
+int dohpMLPNumInputsKind(int kind);
+int dohpMLPNumOutputsKind(int kind);
+dErr dohpMLPCreateKind(int kind, const float *params_host, dohpMLP *mlp);
+/* Sub-cell scales sigma [ncell][3] of a kind-1 network from the features X [ncell][inputs]. */
+dErr dohpMLPCellScales(dohpMLP mlp, long ncell, const float *X_dev, float *scales_dev, void *stream);
+/* One Adam step (beta 0.9/0.999) of a kind-1 network from dL/dsigma [ncell][3] (backpropagated through the
+ * exponential, the clamp and the network). */
+dErr dohpMLPCellStep(dohpMLP mlp, long ncell, const float *X_dev, const float *gsig_dev, float lr, void *stream);
 /* FP32 master parameters and the bfloat16 values the network uses (either may be NULL). */
 dErr dohpMLPGetParams(dohpMLP mlp, float *master_host, float *bf16_values_host);
 dErr dohpMLPDestroy(dohpMLP *mlp);
@@ -344,6 +373,11 @@ dErr dohpEllipSolve(dohpEllip op, dohpMat q1, const double *b_dev, double *x_dev
  * iterate). */
 dErr dohpEllipSolveLearned(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b_dev, double *x_dev,
                            const dohpSolveOpts *opts, dohpSolveStats *stats, void *stream);
+/* The same CG with the multigrid preconditioner on q1 (e.g. the embedded-Q1 matrix assembled with learned sub-cell
+ * scales) until opts->fallback_its iterations have passed without convergence; then the multigrid of q1_fallback
+ * (the plain matrix) takes over and CG restarts from the current iterate (no counterpart in the reference). */
+dErr dohpEllipSolveFallback(dohpEllip op, dohpMat q1, dohpMat q1_fallback, const double *b_dev, double *x_dev,
+                            const dohpSolveOpts *opts, dohpSolveStats *stats, void *stream);
 /* Plain FP64 solve that also records the iterate after every `every`-th iteration into
  * snap_dev [nsnap][nowned] (the residual/error pairs of the learned correction's training). */
 dErr dohpEllipSolveRecord(dohpEllip op, dohpMat q1, const double *b_dev, double *x_dev, const dohpSolveOpts *opts,

@@ -1,8 +1,8 @@
 """TEST INFRASTRUCTURE, NOT PRODUCT CODE.
 
-numpy restatement of the learned node weights of the Q1 multigrid preconditioner: the patch
-features, the network, its node weights and the gradient of a squared-error loss of its raw
-outputs.  The reference has no learned component; these functions define the arithmetic the
+numpy restatement of the learned parts of the Q1 preconditioner: the patch features, the network, its node
+weights, the sub-cell scales of the embedded-Q1 matrix (network kind 1: 11 inputs, 3 outputs) and the gradient
+of a squared-error loss of the raw outputs.  The reference has no learned component; these functions define the arithmetic the
 HIP kernels must reproduce, and the tests pin them by finite differences (gradient) and by
 closed forms (features, bf16 rounding, zero network).
 """
@@ -27,10 +27,25 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
-def unpack(w):
+# sizes per network kind: 0 = patch node weights, 1 = sub-cell scales of the embedded-Q1 matrix
+SHAPES = {0: (NIN, NOUT), 1: (11, 3)}
+
+
+def nparam(kind=0):
+    nin, nout = SHAPES[kind]
+    return NH * nin + NH + NH * NH + NH + nout * NH + nout
+
+
+def unpack(w, kind=0):
+    nin, nout = SHAPES[kind]
     w = np.asarray(w, dtype=np.float64)
-    return (w[OFF_W1:OFF_B1].reshape(NH, NIN), w[OFF_B1:OFF_W2], w[OFF_W2:OFF_B2].reshape(NH, NH),
-            w[OFF_B2:OFF_W3], w[OFF_W3:OFF_B3].reshape(NOUT, NH), w[OFF_B3:])
+    o1 = NH * nin
+    o2 = o1 + NH
+    o3 = o2 + NH * NH
+    o4 = o3 + NH
+    o5 = o4 + nout * NH
+    assert w.size == o5 + nout
+    return (w[:o1].reshape(NH, nin), w[o1:o2], w[o2:o3].reshape(NH, NH), w[o3:o4], w[o4:o5].reshape(nout, NH), w[o5:])
 
 
 def silu(x):
@@ -42,22 +57,23 @@ def dsilu(x):
     return s * (1 + x * (1 - s))
 
 
-def forward(w, X):
-    W1, b1, W2, b2, W3, b3 = unpack(w)
+def forward(w, X, kind=0):
+    W1, b1, W2, b2, W3, b3 = unpack(w, kind)
     Z1 = X @ W1.T + b1
     Z2 = silu(Z1) @ W2.T + b2
     return Z1, Z2, silu(Z2) @ W3.T + b3
 
 
-def loss_and_grad(w, X, T):
-    """Mean squared error of the raw outputs over all B x 8 entries and its gradient (float64)."""
+def loss_and_grad(w, X, T, kind=0):
+    """Mean squared error of the raw outputs over all B x NOUT entries and its gradient (float64)."""
     X = np.asarray(X, dtype=np.float64)
     B = X.shape[0]
-    W1, b1, W2, b2, W3, b3 = unpack(w)
-    Z1, Z2, O = forward(w, X)
+    nout = SHAPES[kind][1]
+    W1, b1, W2, b2, W3, b3 = unpack(w, kind)
+    Z1, Z2, O = forward(w, X, kind)
     D = O - T
     loss = np.mean(D ** 2)
-    dO = 2 * D / (B * NOUT)
+    dO = 2 * D / (B * nout)
     H1, H2 = silu(Z1), silu(Z2)
     dZ2 = (dO @ W3) * dsilu(Z2)
     dZ1 = (dZ2 @ W2) * dsilu(Z1)
@@ -91,3 +107,9 @@ def node_weights(box, w, X):
     s = np.ones(box[0] * box[1] * box[2])
     s[idx] = np.exp(0.5 * np.clip(O, -CLAMP, CLAMP))
     return s
+
+
+def cell_scales(w, X):
+    """Sub-cell derivative scales sigma = exp(clamp(o)) of a kind-1 network (bf16 weights) at features X."""
+    _, _, O = forward(bf16(w), X, kind=1)
+    return np.exp(np.clip(O, -CLAMP, CLAMP))

@@ -12,23 +12,33 @@ from oracle import oracle as O
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared_symbols():
-    syms = set()
-    for fn in os.listdir(os.path.join(ROOT, "include")):
-        if not fn.endswith(".h"):
-            continue
-        txt = open(os.path.join(ROOT, "include", fn)).read()
-        syms |= set(re.findall(r"^\s*(?:dErr|void)\s+(\w+)\s*\(", txt, flags=re.M))
-    return syms
+def _declared_symbols(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return set(re.findall(r"^\s*(?:dErr|void|int)\s+(\w+)\s*\(", txt, flags=re.M))
 
 
 def test_library_exports_every_declared_symbol():
     C = pytest.importorskip("dohp_amd.capi")
-    syms = _declared_symbols()
+    syms = _declared_symbols("dohp_amd.h")
     assert len(syms) >= 15
     lib = C.lib()
     for s in sorted(syms):
         assert hasattr(lib, s), s
+
+
+def test_dhm_library_exports_every_declared_symbol():
+    import ctypes
+    path = os.path.join(ROOT, "dohp_amd", "libdohp_dhm.so")
+    assert os.path.exists(path), "DHM library not built"
+    lib = ctypes.CDLL(path)
+    syms = _declared_symbols("dohp_dhm.h")
+    assert len(syms) >= 10
+    for s in sorted(syms):
+        assert hasattr(lib, s), s
+
+
+def test_every_header_is_covered():
+    assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == ["dohp_amd.h", "dohp_dhm.h"]
 
 
 @pytest.mark.parametrize("method,family,order", [(0, 0, k) for k in range(0, 21)] + [(0, 1, k) for k in range(0, 13)]

@@ -135,3 +135,44 @@ def test_lognormal_poisson_48cube_p5_reaches_1e10(C, lognormal48, mixed):
           f"error vs manufactured {err:.2e}", flush=True)
     assert relres <= 1e-10
     assert err <= 1e-8
+
+
+def test_learned_preconditioner_48cube_p6_beats_plain(C):
+    """Learned sub-cell scales of the embedded-Q1 matrix, trained on residual/error pairs of plain solves of 12^3
+    problems, on the 48^3, p = 6 log-normal problem: fewer CG iterations and less time to 1e-10 than the plain FP64
+    solve, with the plain matrix as the fallback after the plain iteration count."""
+    import time
+    from dohp_amd.learned import ScaledProblem, fit_q1_scales
+    t0 = time.perf_counter()
+    net, rep = fit_q1_scales(C, torch, train_seeds=[101, 102, 103], val_seeds=[201], M=(12, 12, 12), deg=6, epochs=25,
+                             lr=1e-2, npairs=8, every=4, nrandom=2, val_M=(24, 24, 24))
+    print(f"training {time.perf_counter() - t0:.1f}s:", {k: v for k, v in rep.items() if k != "history"}, flush=True)
+    assert net is not None
+    P = ScaledProblem(C, torch, (48, 48, 48), 6, 11)
+
+    def run(q1, **kw):
+        x = torch.zeros_like(P.b)
+        P.op.solve(q1, P.b, x, rtol=1e-10, **kw)  # warm-up: kernels, hierarchy
+        x.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = P.op.solve(q1, P.b, x, rtol=1e-10, **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t
+        r = torch.zeros_like(x)
+        P.op.mat_mult(x, r)
+        relres = float(torch.linalg.norm(P.b - r) / torch.linalg.norm(P.b))
+        err = float(torch.linalg.norm(x - P.xs) / torch.linalg.norm(P.xs))
+        return st["inner_its"], st["fallback"], t, relres, err
+
+    plain = run(P.q1)
+    q1s = P.assemble(net)
+    learned = run(q1s, fallback_q1=P.q1, fallback_its=plain[0])
+    learned32 = run(q1s, fallback_q1=P.q1, fallback_its=plain[0], mixed=2)
+    for name, r in (("plain FP64", plain), ("learned FP64", learned), ("learned FP64 CG + FP32 V-cycle", learned32)):
+        print(f"48^3 p=6 {name}: its {r[0]}, fallback {r[1]}, {r[2]:.3f}s, relres {r[3]:.2e}, error {r[4]:.2e}",
+              flush=True)
+    for r in (plain, learned, learned32):
+        assert r[3] <= 1e-10
+    assert learned[0] < plain[0] and learned[1] == 0
+    assert learned32[2] < plain[2]

@@ -196,3 +196,23 @@ def test_corrected_q1_solves_in_fewer_iterations(C):
             assert relres <= 1.01e-10
             its[(name, mixed)] = st["inner_its"]
     assert its[("corrected", 0)] < its[("reference", 0)]
+
+
+def test_assembly_and_solves_are_reproducible(C):
+    """The embedded-Q1 assembly adds its sub-cell blocks colour by colour (no atomics): two assemblies are bitwise
+    equal.  With the eight-colour scatter of the element kernels every operator application is bitwise reproducible,
+    so repeated solves take the same iterations and give the same iterate, FP32 preconditioner included."""
+    from dohp_amd.learned import make_problem
+    fs, op, q1, b, xs = make_problem(C, (6, 6, 6), 5, 3, torch)
+    q2 = op.assemble_q1(torch.zeros_like(b))
+    A1, A2 = q1.to_scipy(), q2.to_scipy()
+    assert (A1 != A2).nnz == 0
+    op.set_scatter(0)
+    for mixed in (0, 1, 2):
+        out = []
+        for rep in range(2):
+            x = torch.zeros_like(b)
+            st = op.solve(q1, b, x, mixed=mixed, rtol=1e-10)
+            out.append((st["inner_its"], st["outer_its"], x.clone()))
+        print("mixed", mixed, "its", out[0][:2], out[1][:2])
+        assert out[0][:2] == out[1][:2] and torch.equal(out[0][2], out[1][2])

@@ -14,11 +14,13 @@ residual F(u0) and reused by every later one); the time to form it is reported b
 ("quadrature_data_f64"), and the same applications recomputing everything from the state at every
 call ("*_recomputed").  The residual F(u0), the FP32 Jacobian action and the linear (Poisson) apply
 (recomputed by default, from stored data as "linear_*_stored") are timed too and reported under
-"variants".  roofline.achieved divides the algorithmic bytes of one
-J v (x read, y written, the quadrature data read) by the element kernel's own duration (y += J v,
-timed with HIP events on its stream); roofline.traffic
-is the HBM traffic of one launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, read from the
-profile summary committed under profiles/ when it covers this workload.
+"variants".  roofline.achieved divides the algorithmic bytes of one J v, B_Jv(8) of alg_bytes (x and the state
+read, y written, element maps and vertices, geometry recomputed: no stored data), by the element kernel's own duration
+(y += J v, timed with HIP events on its stream); roofline.stored_data_stream gives the same duration against the bytes
+the stored-data form must stream.  roofline.traffic is the HBM traffic of one launch measured with rocprofv3
+FETCH_SIZE/WRITE_SIZE, read from the profile summary committed under profiles/ when it covers this workload.
+compute_roofline divides the reference algorithm's flops of one J v (alg_flops) by the same duration, against the FP64
+peak; every variant carries both fractions (FP32 ones against the FP32 peak).
 
 --workload solve: variable-coefficient Poisson -div(k grad u) = f with log-normal k (unit
 variance, correlation length 0.1, seeded) on 48^3 elements, order 5; CG preconditioned by the
@@ -77,6 +79,33 @@ def alg_bytes(s, n_tot, n_int, n, nelem, nvert, jv):
         B_Jv(s)  = B_lin(s) + s N_tot (read the linearization state u)."""
     b = s * n_tot + s * n_int + 4 * n ** 3 * nelem + 24 * nvert + 32 * nelem
     return b + (s * n_tot if jv else 0)
+
+
+def _contraction_flops(P, Q, D):
+    """Flops of one tensor-product contraction of D components from P^3 to Q^3 values in three passes."""
+    return 2 * D * (Q * P ** 3 + Q ** 2 * P ** 2 + Q ** 3 * P)
+
+
+def alg_flops(n, kind):
+    """Flops per element of one application in the reference's algorithm at n = p + 1 points per direction (FP64,
+    geometry recomputed on every application): a gradient or its transpose is three n^3 -> n^3 contractions (6 n^4
+    flops each), a value interpolation one; the geometry is the Q1 coordinates interpolated and differentiated
+    (four 2^3 -> n^3 contractions of three components) plus the inverse, determinant and weight (44 per point); the
+    push-forward and its transpose 18 per point each; the pointwise step 8 (linear), 20 (Jacobian) or 40 (residual)
+    per point.
+        F_lin = 36 n^4 + (18 + 8 + 18) n^3 + G,   F_Jv = 36 n^4 + (18 + 20 + 18) n^3 + G,
+        F_res = 48 n^4 + (18 + 40 + 18) n^3 + G,  G = 4 C(2 -> n, 3) + 44 n^3."""
+    g = 4 * _contraction_flops(2, n, 3) + 44 * n ** 3
+    if kind == "lin":
+        return 36 * n ** 4 + 44 * n ** 3 + g
+    if kind == "jv":
+        return 36 * n ** 4 + 56 * n ** 3 + g
+    if kind == "res":
+        return 48 * n ** 4 + 76 * n ** 3 + g
+    raise ValueError(kind)
+
+
+FP64_PEAK_TFLOPS, FP32_PEAK_TFLOPS = 78.6, 157.3
 
 
 def stored_bytes(s, n_int, n, nelem, nq=6):
@@ -605,18 +634,21 @@ def main():
     variants = {}
     reps = max(3, args.steps // 2)
 
-    def timed(name, fn, s, jv, nq):
+    def timed(name, fn, s, jv, nq, kind=None):
         t = time_fn(torch, fn, reps)
         b = alg_bytes(s, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, jv)
+        kind = kind or ("jv" if jv else "lin")
+        peak = FP64_PEAK_TFLOPS if s == 8 else FP32_PEAK_TFLOPS
         variants[name] = {"us": round(t * 1e6, 1), "gdofs": round(fs.nowned / t / 1e9, 2),
-                          "hbm_frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}
+                          "hbm_frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4),
+                          "flop_frac": round(alg_flops(n, kind) * fs.nelem / t / 1e12 / peak, 4)}
         if nq:  # what the stored-data form streams, as a fraction of the same peak
             variants[name]["stored_stream_frac"] = round(stored_bytes(s, fs.nowned, n, fs.nelem, nq) / t / 1e9 /
                                                          HBM_PEAK_GBS, 4)
         print(f"# {name} {t * 1e6:.1f} us", file=sys.stderr, flush=True)
         return t
 
-    t_res = timed("residual_f64", lambda: op.function(u, f, stream), 8, True, 0)
+    t_res = timed("residual_f64", lambda: op.function(u, f, stream), 8, True, 0, "res")
     # forming the quadrature data: the first application after a residual pays it
     t_pair = time_fn(torch, lambda: (op.function(u, f, stream), op.mat_mult(v, y, stream)), reps)
     variants["quadrature_data_f64"] = {"us": round((t_pair - t_res - t_step) * 1e6, 1),
@@ -672,6 +704,12 @@ def main():
                 "measured_triad_GBs": round(triad, 1), "frac_vs_measured_triad": round(achieved / triad, 4)}
     if traffic:
         roofline["traffic_over_algorithmic"] = round(traffic / B, 3)
+    F = alg_flops(n, "jv") * fs.nelem
+    compute = {"bound": "fp64", "achieved": round(F / t_k / 1e12, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": round(F / t_k / 1e12 / FP64_PEAK_TFLOPS, 4), "algorithmic_flops": int(F),
+               "flop_model": "F_Jv = 36 n^4 + 56 n^3 + 4 C(2->n, 3) + 44 n^3 per element, C(P->Q, D) = "
+                             "2 D (Q P^3 + Q^2 P^2 + Q^3 P): the reference's gradient, push-forward, pointwise "
+                             "Jacobian, transposes and recomputed geometry"}
     if stored64 == 1:  # the stored form streams its per-point tensor: what it reaches on its own bytes
         sb = stored_bytes(8, fs.nowned, n, fs.nelem)
         roofline["stored_data_stream"] = {"bytes": int(sb), "GBs": round(sb / t_k / 1e9, 1),
@@ -703,6 +741,7 @@ def main():
                    "n_tot": fs.nclosure, "scatter": "colored" if args.scatter == 0 else "atomic",
                    "jacobian_data": "stored per Gauss point (6 FP64 values)" if stored64 == 1 else "recomputed"},
         "roofline": roofline,
+        "compute_roofline": compute,
         "cpu_baseline": cpu,
         "variants": variants,
     }

@@ -108,10 +108,17 @@ class MG:
     def apply_f32(self, r, z, stream=None):
         check(_lib.dohpMGApplyF32(self.h, _ptr(r), _ptr(z), _stream(stream)))
 
-    def residual(self, level, b, x, r, stream=None):
-        """r = b - A_level x (b None: r = -A x); FP32 tensors use the level's FP32 coefficients."""
+    def residual(self, level, b, x, r, stream=None, fp16=False):
+        """r = b - A_level x (b None: r = -A x); FP32 tensors use the level's FP32 coefficients, or with fp16 its FP16
+        ones (the FP32 V-cycle's default)."""
         f = _lib.dohpMGResidualF32 if x.element_size() == 4 else _lib.dohpMGResidual
+        if fp16:
+            f = _lib.dohpMGResidualF16
         check(f(self.h, int(level), None if b is None else _ptr(b), _ptr(x), _ptr(r), _stream(stream)))
+
+    def set_coefficients(self, fp16):
+        """Coefficients of the FP32 V-cycle: FP16 (True, the default) or FP32."""
+        check(_lib.dohpMGSetCoefficients(self.h, int(bool(fp16))))
 
     def __del__(self):
         if _lib is not None and getattr(self, "h", None):
@@ -122,7 +129,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("mixed", ctypes.c_int), ("rtol", ctypes.c_double), ("max_it", ctypes.c_int),
                 ("inner_rtol", ctypes.c_double), ("max_outer", ctypes.c_int), ("cheb_degree", ctypes.c_int),
                 ("cheb_ratio", ctypes.c_double), ("eig_its", ctypes.c_int), ("pc", ctypes.c_int),
-                ("mg_degree", ctypes.c_int), ("fallback_its", ctypes.c_int)]
+                ("mg_degree", ctypes.c_int), ("fallback_its", ctypes.c_int), ("mg_coeffs", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -138,6 +145,8 @@ _proto("dohpMGApply", _vp, _vp, _vp, _vp)
 _proto("dohpMGApplyF32", _vp, _vp, _vp, _vp)
 _proto("dohpMGResidual", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)
 _proto("dohpMGResidualF32", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)
+_proto("dohpMGResidualF16", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)
+_proto("dohpMGSetCoefficients", _vp, ctypes.c_int)
 _proto("dohpMGDestroy", _vp)
 _proto("dohpEllipSolve", _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)
 _proto("dohpEllipSolveLearned", _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), _vp)
@@ -544,15 +553,18 @@ class Ellip:
 
     def solve(self, q1, b, x, mixed=False, rtol=1e-10, max_it=2000, inner_rtol=1e-4, max_outer=50, cheb_degree=4,
               cheb_ratio=30.0, eig_its=20, pc="mg", mg_degree=0, mlp=None, fallback_its=0, record=None,
-              fallback_q1=None, stream=None):
+              fallback_q1=None, mg_coeffs="fp16", stream=None):
         """CG on the linear operator, preconditioned through the embedded-Q1 matrix: pc="mg" is a
         multigrid V-cycle with Jacobi-Chebyshev smoothing, pc="cheb" Jacobi-Chebyshev alone.
         mixed: 0 plain FP64, 1 FP32 inner PCG inside FP64 defect correction, 2 FP64 PCG with the
-        preconditioner in FP32.  mg_degree: smoothing degree, 0 = 1 on a corrected Q1 matrix, 2 otherwise.  mlp: learned node weights around the V-cycle (plain fallback after
+        preconditioner in FP32.  mg_degree: smoothing degree, 0 = 1 on a corrected Q1 matrix or one with learned
+        sub-cell scales, 2 otherwise.  mlp: learned node weights around the V-cycle (plain fallback after
         fallback_its iterations).  fallback_q1: q1 is a learned embedded-Q1 matrix and the multigrid of fallback_q1
-        takes over after fallback_its iterations.  record=(snap, every): plain FP64 solve keeping iterates."""
+        takes over after fallback_its iterations.  mg_coeffs: "fp16" or "fp32" coefficients of the FP32 V-cycle.
+        record=(snap, every): plain FP64 solve keeping iterates."""
         o = SolveOpts(int(mixed), float(rtol), int(max_it), float(inner_rtol), int(max_outer), int(cheb_degree),
-                      float(cheb_ratio), int(eig_its), {"cheb": 0, "mg": 1}[pc], int(mg_degree), int(fallback_its))
+                      float(cheb_ratio), int(eig_its), {"cheb": 0, "mg": 1}[pc], int(mg_degree), int(fallback_its),
+                      {"fp16": 0, "fp32": 1}[mg_coeffs])
         st = SolveStats()
         out = {}
         if fallback_q1 is not None:

@@ -151,6 +151,7 @@ struct _p_dohpMat {
   dohpMG mg = nullptr;
   int mg_degree = 0;
   int corrected = 0;  // assembled with the corrected sub-cell arithmetic (embedded-Q1 mode 1)
+  int scaled = 0;     // assembled with learned per-sub-cell derivative scales
   dohp::DevBuf<double> q1tab;  // the assembly kernel's 1-D tables
   ~_p_dohpMat() { dohpMGDestroy(&mg); }
 };

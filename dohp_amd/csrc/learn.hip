@@ -210,10 +210,11 @@ __global__ void mse_grad_kernel(int B, const float *__restrict__ O, const float 
   sqerr[s] = e2;
 }
 
+// Backward over B samples: dZ2, dZ1 from dO; Z1 and Z2 are overwritten by the activations silu(Z1), silu(Z2), which
+// the parameter gradient reads.
 template <class S>
-__global__ void backward_kernel(int B, const float *__restrict__ wts, const float *__restrict__ Z1,
-                                const float *__restrict__ Z2, const float *__restrict__ dO, float *__restrict__ dZ1,
-                                float *__restrict__ dZ2)
+__global__ void backward_kernel(int B, const float *__restrict__ wts, float *__restrict__ Z1, float *__restrict__ Z2,
+                                const float *__restrict__ dO, float *__restrict__ dZ1, float *__restrict__ dZ2)
 {
   constexpr int NOUT = S::NOUT, OFF_W3 = S::OFF_W3, OFF_W2 = S::OFF_W2;
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -224,46 +225,64 @@ __global__ void backward_kernel(int B, const float *__restrict__ wts, const floa
   for (int j = 0; j < NH; j++) {
     float a = 0.f;
     for (int i = 0; i < NOUT; i++) a = fmaf(w[OFF_W3 + i * NH + j], g[i], a);
-    d2[j] = a * dsilu(Z2[s * NH + j]);
+    const float z = Z2[s * NH + j];
+    d2[j] = a * dsilu(z);
     dZ2[s * NH + j] = d2[j];
+    Z2[s * NH + j] = silu(z);
   }
   for (int j = 0; j < NH; j++) {
     float a = 0.f;
     for (int i = 0; i < NH; i++) a = fmaf(w[OFF_W2 + i * NH + j], d2[i], a);
-    dZ1[s * NH + j] = a * dsilu(Z1[s * NH + j]);
+    const float z = Z1[s * NH + j];
+    dZ1[s * NH + j] = a * dsilu(z);
+    Z1[s * NH + j] = silu(z);
   }
 }
 
-// Parameter gradient, one thread per parameter, summing over samples in order (deterministic).
+// Parameter gradient in two deterministic stages: a (parameter block, sample chunk) grid where each thread sums one
+// parameter's products over its chunk's samples in order into part[chunk][q], then grad_reduce_kernel sums the chunks
+// in order.  A1, A2 are the activations left by backward_kernel.
+constexpr int kGradChunk = 1024;  // samples per chunk
 template <class S>
-__global__ void grad_kernel(int B, const float *__restrict__ X, const float *__restrict__ Z1,
-                            const float *__restrict__ Z2, const float *__restrict__ dO, const float *__restrict__ dZ1,
-                            const float *__restrict__ dZ2, float *__restrict__ grad)
+__global__ void __launch_bounds__(256) grad_kernel(int B, const float *__restrict__ X, const float *__restrict__ A1,
+                                                   const float *__restrict__ A2, const float *__restrict__ dO,
+                                                   const float *__restrict__ dZ1, const float *__restrict__ dZ2,
+                                                   float *__restrict__ part)
 {
   constexpr int NIN = S::NIN, NOUT = S::NOUT, NPARAM = S::NPARAM, OFF_B1 = S::OFF_B1, OFF_W2 = S::OFF_W2,
                 OFF_B2 = S::OFF_B2, OFF_W3 = S::OFF_W3, OFF_B3 = S::OFF_B3;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= NPARAM) return;
+  const long s0 = (long)blockIdx.y * kGradChunk, s1 = s0 + kGradChunk < B ? s0 + kGradChunk : B;
   float acc = 0.f;
   if (q < OFF_B1) {
     const int i = q / NIN, j = q % NIN;
-    for (long s = 0; s < B; s++) acc = fmaf(dZ1[s * NH + i], X[s * NIN + j], acc);
+    for (long s = s0; s < s1; s++) acc = fmaf(dZ1[s * NH + i], X[s * NIN + j], acc);
   } else if (q < OFF_W2) {
     const int i = q - OFF_B1;
-    for (long s = 0; s < B; s++) acc += dZ1[s * NH + i];
+    for (long s = s0; s < s1; s++) acc += dZ1[s * NH + i];
   } else if (q < OFF_B2) {
     const int i = (q - OFF_W2) / NH, j = (q - OFF_W2) % NH;
-    for (long s = 0; s < B; s++) acc = fmaf(dZ2[s * NH + i], silu(Z1[s * NH + j]), acc);
+    for (long s = s0; s < s1; s++) acc = fmaf(dZ2[s * NH + i], A1[s * NH + j], acc);
   } else if (q < OFF_W3) {
     const int i = q - OFF_B2;
-    for (long s = 0; s < B; s++) acc += dZ2[s * NH + i];
+    for (long s = s0; s < s1; s++) acc += dZ2[s * NH + i];
   } else if (q < OFF_B3) {
     const int i = (q - OFF_W3) / NH, j = (q - OFF_W3) % NH;
-    for (long s = 0; s < B; s++) acc = fmaf(dO[s * NOUT + i], silu(Z2[s * NH + j]), acc);
+    for (long s = s0; s < s1; s++) acc = fmaf(dO[s * NOUT + i], A2[s * NH + j], acc);
   } else {
     const int i = q - OFF_B3;
-    for (long s = 0; s < B; s++) acc += dO[s * NOUT + i];
+    for (long s = s0; s < s1; s++) acc += dO[s * NOUT + i];
   }
+  part[(long)blockIdx.y * NPARAM + q] = acc;
+}
+
+__global__ void grad_reduce_kernel(int nparam, int nchunk, const float *__restrict__ part, float *__restrict__ grad)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nparam) return;
+  float acc = 0.f;
+  for (int c = 0; c < nchunk; c++) acc += part[(long)c * nparam + q];
   grad[q] = acc;
 }
 
@@ -361,6 +380,7 @@ struct _p_dohpMLP {
   int step = 0;
   long B = 0;  // capacity of the per-sample buffers
   DevBuf<float> Z1, Z2, O, dO, dZ1, dZ2, sqerr;
+  DevBuf<float> part;  // per-chunk partial parameter gradients
 };
 
 namespace {
@@ -376,6 +396,7 @@ dErr ensure(_p_dohpMLP *n, long B)
   DOHP_CALL(n->O.alloc((size_t)B * nout_of(n->kind)));
   DOHP_CALL(n->dO.alloc((size_t)B * nout_of(n->kind)));
   DOHP_CALL(n->sqerr.alloc(B));
+  DOHP_CALL(n->part.alloc((size_t)((B + kGradChunk - 1) / kGradChunk) * NPARAM_MAX));
   return 0;
 }
 
@@ -384,8 +405,12 @@ dErr backward_and_grad_s(_p_dohpMLP *n, long B, const float *X, hipStream_t st)
 {
   hipLaunchKernelGGL(backward_kernel<S>, grid_of(B, 128), dim3(128), 0, st, (int)B, n->wts.ptr, n->Z1.ptr, n->Z2.ptr,
                      n->dO.ptr, n->dZ1.ptr, n->dZ2.ptr);
-  hipLaunchKernelGGL(grad_kernel<S>, grid_of(S::NPARAM), dim3(256), 0, st, (int)B, X, n->Z1.ptr, n->Z2.ptr, n->dO.ptr,
-                     n->dZ1.ptr, n->dZ2.ptr, n->grad.ptr);
+  const int nchunk = (int)((B + kGradChunk - 1) / kGradChunk);
+  DOHP_CHECK(nchunk <= 65535, dohpERR_ARG, "too many samples for one gradient");
+  hipLaunchKernelGGL(grad_kernel<S>, dim3((S::NPARAM + 255) / 256, nchunk), dim3(256), 0, st, (int)B, X, n->Z1.ptr,
+                     n->Z2.ptr, n->dO.ptr, n->dZ1.ptr, n->dZ2.ptr, n->part.ptr);
+  hipLaunchKernelGGL(grad_reduce_kernel, grid_of(S::NPARAM), dim3(256), 0, st, S::NPARAM, nchunk, n->part.ptr,
+                     n->grad.ptr);
   DOHP_HIP(hipGetLastError());
   return 0;
 }

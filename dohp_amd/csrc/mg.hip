@@ -43,6 +43,11 @@ __device__ __forceinline__ bool box_index(const Box &b, int &I, int &J, int &K, 
   return true;
 }
 
+__device__ __forceinline__ bool inside(const Box &b, int I, int J, int K)
+{
+  return I >= 0 && I < b.nx && J >= 0 && J < b.ny && K >= 0 && K < b.nz;
+}
+
 __device__ __forceinline__ void unrank(const Box &b, long r, int &I, int &J, int &K)
 {
   const long yz = (long)b.ny * b.nz;
@@ -103,12 +108,15 @@ __global__ void stencil_mult_kernel(Box b, const T *__restrict__ A, const T *__r
   long r;
   if (!box_index(b, I, J, K, r)) return;
   const long sy = b.nz, sx = (long)b.ny * b.nz;
+  // every load is issued unconditionally (a neighbour outside the box reads the row's own x and is dropped by a
+  // select), so the 27 loads of a row are in flight together instead of one branch and wait each
   T acc = 0;
 #pragma unroll
   for (int s = 0; s < 27; s++) {
     const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
-    if (I + di < 0 || I + di >= b.nx || J + dj < 0 || J + dj >= b.ny || K + dk < 0 || K + dk >= b.nz) continue;
-    acc += A[s * n + r] * x[r + di * sx + dj * sy + dk];
+    const bool in = inside(b, I + di, J + dj, K + dk);
+    const T v = A[s * n + r] * x[in ? r + di * sx + dj * sy + dk : r];
+    acc += in ? v : T(0);
   }
   y[r] = acc;
 }
@@ -126,15 +134,17 @@ __global__ void __launch_bounds__(256) half_residual_kernel(Box bx, const T *__r
   long i;
   if (!box_index(bx, I, J, K, i)) return;
   const long sy = bx.nz, sx = (long)bx.ny * bx.nz;
+  // unconditional loads at clamped indices, dropped by selects (see stencil_mult_kernel)
   T acc = H[i] * x[i];
 #pragma unroll
   for (int s = 14; s < 27; s++) {
     const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
     const long off = di * sx + dj * sy + dk;
-    if (I + di >= 0 && I + di < bx.nx && J + dj >= 0 && J + dj < bx.ny && K + dk >= 0 && K + dk < bx.nz)
-      acc += H[(s - 13) * n + i] * x[i + off];
-    if (I - di >= 0 && I - di < bx.nx && J - dj >= 0 && J - dj < bx.ny && K - dk >= 0 && K - dk < bx.nz)
-      acc += H[(s - 13) * n + i - off] * x[i - off];
+    const bool up = inside(bx, I + di, J + dj, K + dk), lo = inside(bx, I - di, J - dj, K - dk);
+    const long iu = up ? i + off : i, il = lo ? i - off : i;
+    const T vu = H[(s - 13) * n + i] * x[iu], vl = H[(s - 13) * n + il] * x[il];
+    acc += up ? vu : T(0);
+    acc += lo ? vl : T(0);
   }
   r[i] = (b ? b[i] : T(0)) - acc;
 }
@@ -144,6 +154,113 @@ template <typename T>
 void residual(const Box &bx, const T *H, const T *b, const T *x, T *r, hipStream_t st)
 {
   hipLaunchKernelGGL(half_residual_kernel<T>, box_grid(bx), box_block(), 0, st, bx, H, b, x, r);
+}
+
+// FP16 coefficients of the FP32 V-cycle.  With the row scales s_i = sqrt(a_ii) (FP32), a level keeps the 13 upper
+// off-diagonals as h_ij = fp16(a_ij / (s_i s_j)) (|h_ij| <= 1 for a symmetric positive definite matrix) and, in FP32,
+// the diagonal d_i = a_ii + sum_j (a_ij - s_i s_j h_ij) over all 26 neighbours: the rounding of the off-diagonals
+// moves into the diagonal, so every row sum (and with it the near-null space of constants, on which the
+// preconditioner's quality rests) is that of the FP64 matrix.  A row then reads 26 + 8 bytes of coefficients
+// instead of 56.
+__device__ __forceinline__ _Float16 coeff16(double a, float si, float sj)
+{
+  return (_Float16)(float)(a / ((double)si * (double)sj));
+}
+
+__global__ void __launch_bounds__(256) half16_setup_kernel(Box bx, const double *__restrict__ A,
+                                                           _Float16 *__restrict__ H, float *__restrict__ dg,
+                                                           float *__restrict__ sc)
+{
+  const long n = bx.n();
+  int I, J, K;
+  long i;
+  if (!box_index(bx, I, J, K, i)) return;
+  const long sy = bx.nz, sx = (long)bx.ny * bx.nz;
+  const float si = (float)sqrt(A[13 * n + i]);
+  double comp = 0;
+#pragma unroll 1
+  for (int s = 14; s < 27; s++) {
+    const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
+    const long off = di * sx + dj * sy + dk;
+    _Float16 h = 0;
+    if (I + di >= 0 && I + di < bx.nx && J + dj >= 0 && J + dj < bx.ny && K + dk >= 0 && K + dk < bx.nz) {
+      const float sj = (float)sqrt(A[13 * n + i + off]);
+      const double a = A[s * n + i];
+      h = coeff16(a, si, sj);
+      comp += a - (double)si * (double)sj * (double)h;
+    }
+    H[(s - 14) * n + i] = h;
+    if (I - di >= 0 && I - di < bx.nx && J - dj >= 0 && J - dj < bx.ny && K - dk >= 0 && K - dk < bx.nz) {
+      // the lower neighbour's entry is its upper one, rounded as that row rounds it
+      const float sj = (float)sqrt(A[13 * n + i - off]);
+      const double a = A[s * n + i - off];
+      const _Float16 hl = coeff16(a, sj, si);
+      comp += a - (double)sj * (double)si * (double)hl;
+    }
+  }
+  dg[i] = (float)(A[13 * n + i] + comp);
+  sc[i] = si;
+}
+
+// (A x)_i on the FP16 coefficients, accumulated in FP32.
+__device__ __forceinline__ float half16_row(const Box &bx, const _Float16 *__restrict__ H, const float *__restrict__ dg,
+                                            const float *__restrict__ sc, const float *__restrict__ x, int I, int J,
+                                            int K, long i)
+{
+  const long n = bx.n(), sy = bx.nz, sx = (long)bx.ny * bx.nz;
+  // unconditional loads at clamped indices, dropped by selects (see stencil_mult_kernel)
+  float acc = 0;
+#pragma unroll
+  for (int s = 14; s < 27; s++) {
+    const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
+    const long off = di * sx + dj * sy + dk;
+    const bool up = inside(bx, I + di, J + dj, K + dk), lo = inside(bx, I - di, J - dj, K - dk);
+    const long iu = up ? i + off : i, il = lo ? i - off : i;
+    const float vu = (float)H[(s - 14) * n + i] * (sc[iu] * x[iu]);
+    const float vl = (float)H[(s - 14) * n + il] * (sc[il] * x[il]);
+    acc += up ? vu : 0.f;
+    acc += lo ? vl : 0.f;
+  }
+  return dg[i] * x[i] + sc[i] * acc;
+}
+
+__global__ void __launch_bounds__(256) half16_residual_kernel(Box bx, const _Float16 *__restrict__ H,
+                                                              const float *__restrict__ dg, const float *__restrict__ sc,
+                                                              const float *__restrict__ b, const float *__restrict__ x,
+                                                              float *__restrict__ r)
+{
+  int I, J, K;
+  long i;
+  if (!box_index(bx, I, J, K, i)) return;
+  r[i] = (b ? b[i] : 0.f) - half16_row(bx, H, dg, sc, x, I, J, K, i);
+}
+
+// One Chebyshev step of the smoother on the FP16 coefficients, out of place (Jacobi needs the old iterate at the
+// neighbours): dn = a d + c (b - A xin) / d_ii, xout = xin + dn; d is read when a != 0 and written when keep_d.
+__global__ void __launch_bounds__(256) half16_cheb_kernel(Box bx, const _Float16 *__restrict__ H,
+                                                          const float *__restrict__ dg, const float *__restrict__ sc,
+                                                          const float *__restrict__ b, const float *__restrict__ xin,
+                                                          float *__restrict__ xout, float *__restrict__ d, float a,
+                                                          float c, int keep_d)
+{
+  int I, J, K;
+  long i;
+  if (!box_index(bx, I, J, K, i)) return;
+  const float r = b[i] - half16_row(bx, H, dg, sc, xin, I, J, K, i);
+  const float dn = (a != 0.f ? a * d[i] : 0.f) + c * r / dg[i];
+  if (keep_d) d[i] = dn;
+  xout[i] = xin[i] + dn;
+}
+
+// First step from a zero iterate: x = d = c b / d_ii.
+__global__ void half16_cheb_first_kernel(long n, float c, const float *__restrict__ dg, const float *__restrict__ b,
+                                         float *__restrict__ x, float *__restrict__ d, int keep_d)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float dn = c * b[i] / dg[i];
+  if (keep_d) d[i] = dn;
+  x[i] = dn;
 }
 
 __global__ void half_from_full_kernel(long n, const double *__restrict__ A, double *__restrict__ H,
@@ -200,19 +317,56 @@ __global__ void restrict_kernel(Box f, Box c, const T *__restrict__ rf, T *__res
   if (!box_index(c, I, J, K, C)) return;
   const long sy = f.nz, sx = (long)f.ny * f.nz;
   const int F0 = 2 * I + 1, F1 = 2 * J + 1, F2 = 2 * K + 1;
+  const long base = F0 * sx + F1 * sy + F2;  // (F0, F1, F2) is inside the fine box; its upper neighbours may not be
   T acc = 0;
-  for (int a = -1; a <= 1; a++) {
-    if (F0 + a >= f.nx) continue;
-    for (int b = -1; b <= 1; b++) {
-      if (F1 + b >= f.ny) continue;
+#pragma unroll
+  for (int a = -1; a <= 1; a++)
+#pragma unroll
+    for (int b = -1; b <= 1; b++)
+#pragma unroll
       for (int d = -1; d <= 1; d++) {
-        if (F2 + d >= f.nz) continue;
+        const bool in = F0 + a < f.nx && F1 + b < f.ny && F2 + d < f.nz;
         const T w = T((a ? 0.5 : 1.0) * (b ? 0.5 : 1.0) * (d ? 0.5 : 1.0));
-        acc += w * rf[(F0 + a) * sx + (F1 + b) * sy + F2 + d];
+        const T v = w * rf[in ? base + a * sx + b * sy + d : base];  // unconditional load, dropped by a select
+        acc += in ? v : T(0);
       }
-    }
-  }
   bc[C] = acc;
+}
+
+// The two coarse parents of fine index f in one direction with their weights; a missing parent has weight 0 and
+// index 0, so that prolongations load unconditionally (nc >= 1).
+__device__ __forceinline__ void parents2(int f, int nc, int (&c)[2], float (&w)[2])
+{
+  int cc[2];
+  double ww[2];
+  const int m = parents(f, nc, cc, ww);
+  c[0] = m > 0 ? cc[0] : 0;
+  w[0] = m > 0 ? (float)ww[0] : 0.f;
+  c[1] = m > 1 ? cc[1] : 0;
+  w[1] = m > 1 ? (float)ww[1] : 0.f;
+}
+
+// xout = xin + P xc (out of place: the FP16 V-cycle's iterate alternates between two buffers)
+__global__ void prolong_kernel(Box f, Box c, const float *__restrict__ xc, const float *__restrict__ xin,
+                               float *__restrict__ xout)
+{
+  int I, J, K;
+  long F;
+  if (!box_index(f, I, J, K, F)) return;
+  int ci[2], cj[2], ck[2];
+  float wi[2], wj[2], wk[2];
+  parents2(I, c.nx, ci, wi);
+  parents2(J, c.ny, cj, wj);
+  parents2(K, c.nz, ck, wk);
+  float acc = 0;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int d = 0; d < 2; d++)
+        acc += (wi[a] * wj[b] * wk[d]) * xc[((long)ci[a] * c.ny + cj[b]) * c.nz + ck[d]];
+  xout[F] = xin[F] + acc;
 }
 
 // xf += P xc
@@ -223,12 +377,17 @@ __global__ void prolong_add_kernel(Box f, Box c, const T *__restrict__ xc, T *__
   long F;
   if (!box_index(f, I, J, K, F)) return;
   int ci[2], cj[2], ck[2];
-  double wi[2], wj[2], wk[2];
-  const int mi = parents(I, c.nx, ci, wi), mj = parents(J, c.ny, cj, wj), mk = parents(K, c.nz, ck, wk);
+  float wi[2], wj[2], wk[2];  // 0, 1/2 or 1: their products are exact in either precision
+  parents2(I, c.nx, ci, wi);
+  parents2(J, c.ny, cj, wj);
+  parents2(K, c.nz, ck, wk);
   T acc = 0;
-  for (int a = 0; a < mi; a++)
-    for (int b = 0; b < mj; b++)
-      for (int d = 0; d < mk; d++)
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int d = 0; d < 2; d++)
         acc += T(wi[a] * wj[b] * wk[d]) * xc[((long)ci[a] * c.ny + cj[b]) * c.nz + ck[d]];
   xf[F] += acc;
 }
@@ -376,6 +535,8 @@ struct MGLevel {
   const double *A64 = nullptr;  // full 27-point stencil (finest level: the Q1 matrix's own)
   DevBuf<double> own64, H64, dinv64, b64, x64, r64, d64;  // H: symmetric half stencil [14][n]
   DevBuf<float> H32, dinv32, b32, x32, r32, d32;
+  DevBuf<_Float16> H16;        // FP16 upper off-diagonals [13][n] (see half16_setup_kernel)
+  DevBuf<float> dg16, sc16, t32;  // their diagonal and row scales; the second iterate buffer of the FP16 V-cycle
   double lmax = 0;
 };
 
@@ -384,6 +545,7 @@ struct _p_dohpMG {
   int degree = 2;
   double lo = 0.1, hi = 1.1;
   int ncoarse = 0;
+  int coeff16 = 1;  // the FP32 V-cycle reads the FP16 coefficients
   DevBuf<double> cinv64;
   DevBuf<float> cinv32;
 };
@@ -518,6 +680,59 @@ void smooth(const _p_dohpMG *mg, MGLevel &lv, const T *b, T *x, bool zero_init, 
   }
 }
 
+// The FP32 smoother on the FP16 coefficients: Chebyshev steps alternate the iterate between cur and other (zero_init:
+// cur is first set from b).  Returns the buffer that holds the result.
+float *smooth16(const _p_dohpMG *mg, MGLevel &lv, const float *b, float *cur, float *other, bool zero_init,
+                hipStream_t st)
+{
+  const double lmin = mg->lo * lv.lmax, lmaxs = mg->hi * lv.lmax;
+  const double theta = 0.5 * (lmaxs + lmin), delta = 0.5 * (lmaxs - lmin), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  const int keep_d = mg->degree > 1;
+  if (zero_init) {
+    hipLaunchKernelGGL(half16_cheb_first_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, (float)(1.0 / theta),
+                       lv.dg16.ptr, b, cur, lv.d32.ptr, keep_d);
+  } else {
+    hipLaunchKernelGGL(half16_cheb_kernel, box_grid(lv.box), box_block(), 0, st, lv.box, lv.H16.ptr, lv.dg16.ptr,
+                       lv.sc16.ptr, b, cur, other, lv.d32.ptr, 0.f, (float)(1.0 / theta), keep_d);
+    std::swap(cur, other);
+  }
+  for (int k = 1; k < mg->degree; k++) {
+    const double rho_new = 1.0 / (2.0 * sigma - rho);
+    hipLaunchKernelGGL(half16_cheb_kernel, box_grid(lv.box), box_block(), 0, st, lv.box, lv.H16.ptr, lv.dg16.ptr,
+                       lv.sc16.ptr, b, cur, other, lv.d32.ptr, (float)(rho_new * rho), (float)(2.0 * rho_new / delta),
+                       keep_d);
+    std::swap(cur, other);
+    rho = rho_new;
+  }
+  return cur;
+}
+
+// FP32 V-cycle on the FP16 coefficients.  The iterate alternates between x and the level's second buffer: the
+// pre-smoother's steps, the prolongation and the post-smoother's steps make an even number of moves, so the result
+// ends in x.
+void vcycle16(_p_dohpMG *mg, size_t l, const float *b, float *x, hipStream_t st)
+{
+  MGLevel &lv = mg->L[l];
+  if (l + 1 == mg->L.size()) {
+    hipLaunchKernelGGL(dense_mv_kernel<float>, dim3((unsigned)((lv.n + 3) / 4)), dim3(256), 0, st, (int)lv.n,
+                       mg->cinv32.ptr, b, x);
+    return;
+  }
+  MGLevel &cv = mg->L[l + 1];
+  float *cur = smooth16(mg, lv, b, x, lv.t32.ptr, true, st);
+  hipLaunchKernelGGL(half16_residual_kernel, box_grid(lv.box), box_block(), 0, st, lv.box, lv.H16.ptr, lv.dg16.ptr,
+                     lv.sc16.ptr, b, cur, lv.r32.ptr);
+  hipLaunchKernelGGL(restrict_kernel<float>, box_grid(cv.box), box_block(), 0, st, lv.box, cv.box, lv.r32.ptr,
+                     cv.b32.ptr);
+  vcycle16(mg, l + 1, cv.b32.ptr, cv.x32.ptr, st);
+  float *other = cur == x ? lv.t32.ptr : x;
+  hipLaunchKernelGGL(prolong_kernel, box_grid(lv.box), box_block(), 0, st, lv.box, cv.box, cv.x32.ptr, cur, other);
+  cur = smooth16(mg, lv, b, other, cur, false, st);
+  if (cur != x)  // (not reached: the moves are even)
+    (void)hipMemcpyAsync(x, cur, sizeof(float) * lv.n, hipMemcpyDeviceToDevice, st);
+}
+
 template <typename T>
 void vcycle(_p_dohpMG *mg, size_t l, const T *b, T *x, hipStream_t st)
 {
@@ -585,8 +800,15 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
     if (!err && l > 0) err = lv.x64.alloc(n);
     if (!err && l > 0) err = lv.b32.alloc(n);
     if (!err && l > 0) err = lv.x32.alloc(n);
+    if (!err && l + 1 < mg->L.size()) err = lv.H16.alloc(13 * n);
+    if (!err && l + 1 < mg->L.size()) err = lv.dg16.alloc(n);
+    if (!err && l + 1 < mg->L.size()) err = lv.sc16.alloc(n);
+    if (!err && l + 1 < mg->L.size()) err = lv.t32.alloc(n);
     if (err) break;
     hipLaunchKernelGGL(half_from_full_kernel, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.H64.ptr, lv.H32.ptr);
+    if (l + 1 < mg->L.size())
+      hipLaunchKernelGGL(half16_setup_kernel, box_grid(lv.box), box_block(), 0, st, lv.box, lv.A64, lv.H16.ptr,
+                         lv.dg16.ptr, lv.sc16.ptr);
     hipLaunchKernelGGL(diag_inv_kernel<double>, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.dinv64.ptr);
     hipLaunchKernelGGL(to_f32<float>, grid_of(n), dim3(256), 0, st, n, lv.dinv64.ptr, lv.dinv32.ptr);
     if (l + 1 < mg->L.size()) err = estimate_lmax(lv, st, 15);
@@ -647,7 +869,10 @@ extern "C" dErr dohpMGApply(dohpMG mg, const double *r, double *z, void *stream)
 extern "C" dErr dohpMGApplyF32(dohpMG mg, const float *r, float *z, void *stream)
 {
   DOHP_CHECK(mg && r && z, dohpERR_ARG, "null argument");
-  vcycle<float>(mg, 0, r, z, (hipStream_t)stream);
+  if (mg->coeff16 && mg->L.size() > 1)
+    vcycle16(mg, 0, r, z, (hipStream_t)stream);
+  else
+    vcycle<float>(mg, 0, r, z, (hipStream_t)stream);
   DOHP_HIP(hipGetLastError());
   return 0;
 }
@@ -669,6 +894,25 @@ extern "C" dErr dohpMGResidualF32(dohpMG mg, int level, const float *b, const fl
   const MGLevel &lv = mg->L[level];
   residual<float>(lv.box, lv.H32.ptr, b, x, r, (hipStream_t)stream);
   DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" dErr dohpMGResidualF16(dohpMG mg, int level, const float *b, const float *x, float *r, void *stream)
+{
+  DOHP_CHECK(mg && level >= 0 && level + 1 < (int)mg->L.size(), dohpERR_ARG,
+             "no FP16 coefficients on this multigrid level (the coarsest is solved directly)");
+  DOHP_CHECK(x && r, dohpERR_ARG, "null argument");
+  const MGLevel &lv = mg->L[level];
+  hipLaunchKernelGGL(half16_residual_kernel, box_grid(lv.box), box_block(), 0, (hipStream_t)stream, lv.box,
+                     lv.H16.ptr, lv.dg16.ptr, lv.sc16.ptr, b, x, r);
+  DOHP_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" dErr dohpMGSetCoefficients(dohpMG mg, int fp16)
+{
+  DOHP_CHECK(mg && (fp16 == 0 || fp16 == 1), dohpERR_ARG, "coefficients must be 0 (FP32) or 1 (FP16)");
+  mg->coeff16 = fp16;
   return 0;
 }
 

@@ -632,6 +632,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   tb.n = fs->n;
   tb.corrected = q1mode;
   A->corrected = q1mode;
+  A->scaled = cscale != nullptr;
   {
     double zg[kMaxN], wg[kMaxN];
     gauss_lobatto(fs->n, zg, wg);

@@ -344,7 +344,7 @@ namespace {
 // Multigrid hierarchy of q1 at smoothing degree `deg` (0 = the default for its arithmetic), kept with the matrix.
 dErr hierarchy(dohpMat q1, int deg, hipStream_t st, dohpMG *mg)
 {
-  deg = deg > 0 ? deg : (q1->corrected ? 1 : 2);
+  deg = deg > 0 ? deg : (q1->corrected || q1->scaled ? 1 : 2);
   if (!q1->mg || q1->mg_degree != deg) {
     dohpMGDestroy(&q1->mg);
     DOHP_CALL(dohpMGCreate(q1, deg, 0, &q1->mg, st));
@@ -392,10 +392,14 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     dohpMG mg = nullptr, mg_fb = nullptr;
   } mgh;
   if (opts->pc == 1) {  // the hierarchy is kept with the matrix until it is refilled
-    // default smoothing degree: 1 on the corrected Q1 matrix (fastest, measured), 2 on the reference arithmetic's
-    // (where degree 1 with the FP32 V-cycle can stall)
+    // default smoothing degree: 1 on the corrected Q1 matrix and on one with learned sub-cell scales (fastest,
+    // measured), 2 on the reference arithmetic's (where degree 1 with the FP32 V-cycle can stall); a fallback
+    // matrix gets its own default
     DOHP_CALL(hierarchy(q1, opts->mg_degree, c.st, &mgh.mg));
     if (q1fb) DOHP_CALL(hierarchy(q1fb, opts->mg_degree, c.st, &mgh.mg_fb));
+    DOHP_CHECK(opts->mg_coeffs == 0 || opts->mg_coeffs == 1, dohpERR_ARG, "mg_coeffs must be 0 (FP16) or 1 (FP32)");
+    DOHP_CALL(dohpMGSetCoefficients(mgh.mg, opts->mg_coeffs == 0));
+    if (mgh.mg_fb) DOHP_CALL(dohpMGSetCoefficients(mgh.mg_fb, opts->mg_coeffs == 0));
   }
   Budget bud;
   bud.fallback_its = mlp || q1fb ? opts->fallback_its : 0;

@@ -276,6 +276,12 @@ dErr dohpMGApplyF32(dohpMG mg, const float *r_dev, float *z_dev, void *stream);
  * MatResidual on the level operator.  r must not alias b or x. */
 dErr dohpMGResidual(dohpMG mg, int level, const double *b_dev, const double *x_dev, double *r_dev, void *stream);
 dErr dohpMGResidualF32(dohpMG mg, int level, const float *b_dev, const float *x_dev, float *r_dev, void *stream);
+/* The FP32 V-cycle's coefficients (no counterpart in the reference, whose preconditioner is FP64): fp16 = 1 (the
+ * default) reads the symmetrically scaled off-diagonals of every smoothed level in FP16 and a diagonal that keeps the
+ * FP64 row sums; fp16 = 0 reads FP32 copies.  dohpMGResidualF16 is the residual on the FP16 coefficients (levels
+ * above the coarsest). */
+dErr dohpMGSetCoefficients(dohpMG mg, int fp16);
+dErr dohpMGResidualF16(dohpMG mg, int level, const float *b_dev, const float *x_dev, float *r_dev, void *stream);
 dErr dohpMGDestroy(dohpMG *mg);
 
 /* ------------------------------------------- learned correction of the Q1 preconditioner */
@@ -348,9 +354,12 @@ typedef struct {
   int pc;             /* 0: Jacobi-Chebyshev on the Q1 matrix; 1: multigrid V-cycle on it (dohpMG) */
   int mg_degree;      /* Chebyshev degree of the multigrid smoother (pre and post).  0: 1 on a corrected
                          embedded-Q1 matrix (dohpEllipSetQ1Mode 1; 48^3, p = 5: 0.08 s against 0.14 s at
-                         degree 2), 2 on the reference arithmetic's, where degree 1 with an FP32
-                         preconditioner can stall (48^3, p = 6: 489 iterations against 55) */
+                         degree 2) and on one assembled with learned sub-cell scales (48^3, p = 6: 42 iterations
+                         against 51), 2 on the reference arithmetic's, where degree 1 with an FP32
+                         preconditioner can stall (48^3, p = 6: 400 iterations without reaching 1e-10, against 100) */
   int fallback_its;   /* learned solve: iterations after which the plain preconditioner takes over (0 = never) */
+  int mg_coeffs;      /* coefficients of the FP32 V-cycle (mixed = 1, 2): 0 = FP16 off-diagonals with a
+                         row-sum-preserving FP32 diagonal (dohpMGSetCoefficients), 1 = FP32 */
 } dohpSolveOpts;
 
 typedef struct {

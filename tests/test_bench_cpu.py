@@ -39,3 +39,40 @@ def test_committed_traffic_line_is_read():
     assert bench.measured_traffic("nonlinear diffusion J(u0)v, 64^3 hex, p=6, FP64, perturbed mesh") > 1e9
     assert bench.measured_traffic(
         "nonlinear diffusion J(u0)v from stored quadrature data, 64^3 hex, p=6, FP64, perturbed mesh") > 5e9
+
+
+def test_headline_flop_model():
+    # the three-pass contraction counted by running it: every multiply-add of each pass is counted
+    import numpy as np
+
+    def counted(P, Q, D):
+        A = [np.ones((Q, P))] * 3
+        u = np.ones((P, P, P, D))
+        macs = 0
+        a = np.zeros((Q, P, P, D))
+        for i in range(Q):
+            for m in range(P):
+                a[i] += A[0][i, m] * u[m]
+                macs += P * P * D
+        b = np.zeros((Q, Q, P, D))
+        for j in range(Q):
+            for m in range(P):
+                b[:, j] += A[1][j, m] * a[:, m]
+                macs += Q * P * D
+        c = np.zeros((Q, Q, Q, D))
+        for k in range(Q):
+            for m in range(P):
+                c[:, :, k] += A[2][k, m] * b[:, :, m]
+                macs += Q * Q * D
+        assert c[0, 0, 0, 0] == P ** 3
+        return 2 * macs
+
+    for P, Q, D in ((2, 7, 3), (7, 7, 1), (3, 5, 2)):
+        assert bench._contraction_flops(P, Q, D) == counted(P, Q, D)
+    # the reference's figures at p = 6: 14,406 flops per TensorMult_Hex call, about 22.5 kflop of coordinate
+    # contractions per element for the geometry
+    assert bench._contraction_flops(7, 7, 1) == 14406
+    assert abs(4 * bench._contraction_flops(2, 7, 3) - 22.5e3) < 0.1e3
+    # J v at p = 6: six gradient passes, push-forwards, pointwise, geometry = 143,248 flops per element
+    assert bench.alg_flops(7, "jv") == 6 * 14406 + 56 * 343 + 4 * bench._contraction_flops(2, 7, 3) + 44 * 343
+    assert bench.alg_flops(7, "jv") == 143248

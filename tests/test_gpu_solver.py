@@ -121,6 +121,43 @@ def test_level_residual_matches_the_stencil_product(C, M, deg):
         np.testing.assert_allclose(r32.cpu().numpy(), bh - A @ xh, rtol=0, atol=2e-6 * scale)
 
 
+@pytest.mark.parametrize("M,deg", [((6, 7, 5), 5), ((3, 4, 14), 5)])
+def test_fp16_level_coefficients(C, M, deg):
+    """The FP32 V-cycle's FP16 coefficients on every smoothed level: the residual is within the FP16 rounding of the
+    off-diagonals of the FP64 one; the rows of ones give the FP64 row sums to FP32 rounding (the off-diagonals'
+    rounding moved into the diagonal: without it the row sums would be off by about 1e-3 of the row's magnitude);
+    the operator is symmetric to FP32 rounding."""
+    print(f"fp16 level coefficients M={M} p={deg}", flush=True)
+    fs, op, q1, xs, b = _problem(C, M, deg)
+    mg = C.MG(q1, smooth_degree=2, max_coarse=64)
+    assert mg.nlevels >= 2
+    for l in range(mg.nlevels - 1):
+        box, st = mg.level(l)
+        A = _stencil_to_csr(box, st)
+        n = A.shape[0]
+        rowabs = np.asarray(abs(A).sum(axis=1)).ravel()
+        x = torch.as_tensor(splitmix64_uniform(11 + l, n) - 0.5, dtype=torch.float32, device="cuda")
+        y = torch.as_tensor(splitmix64_uniform(71 + l, n) - 0.5, dtype=torch.float32, device="cuda")
+        bb = torch.as_tensor(splitmix64_uniform(31 + l, n) - 0.5, dtype=torch.float32, device="cuda")
+        xh, yh, bh = (t.double().cpu().numpy() for t in (x, y, bb))
+        scale = (abs(A) @ np.abs(xh)).max()
+        r = torch.full_like(x, np.nan)
+        mg.residual(l, bb, x, r, fp16=True)
+        err = np.abs(r.double().cpu().numpy() - (bh - A @ xh)).max()
+        print(f"  level {l} box {box}: residual error {err / scale:.2e} of max |A||x|")
+        assert err <= 1e-3 * scale
+        ones = torch.ones_like(x)
+        mg.residual(l, None, ones, r, fp16=True)
+        rs = np.abs(r.double().cpu().numpy() + A @ np.ones(n)).max()
+        print(f"  level {l}: row sums off by {rs / rowabs.max():.2e} of max_i sum_j |a_ij|")
+        assert rs <= 2e-6 * rowabs.max()
+        ry = torch.empty_like(y)
+        mg.residual(l, None, x, r, fp16=True)
+        mg.residual(l, None, y, ry, fp16=True)
+        lhs, rhs = float(r.double() @ y.double()), float(x.double() @ ry.double())
+        assert abs(lhs - rhs) <= 1e-6 * float(np.abs(xh) @ (abs(A) @ np.abs(yh)))
+
+
 @pytest.mark.parametrize("M,deg", [((6, 6, 6), 5), ((4, 5, 3), 3)])
 def test_solve_plain_and_mixed_reach_1e10(C, M, deg):
     print(f"solve M={M} p={deg}", flush=True)
@@ -141,6 +178,25 @@ def test_solve_plain_and_mixed_reach_1e10(C, M, deg):
     assert its[("mg", 0)] < its[("cheb", 0)]
     # an FP32 preconditioner inside FP64 CG costs at most a few iterations
     assert its[("mg", 2)] <= its[("mg", 0)] + 3
+
+
+def test_fp16_and_fp32_vcycle_coefficients_solve_alike(C):
+    """FP64 CG with the FP32 V-cycle, and FP32-inner/FP64-outer defect correction, reach 1e-10 with the V-cycle on
+    FP16 coefficients (the default) in as many iterations as on FP32 ones, up to two."""
+    print("fp16 vs fp32 V-cycle coefficients", flush=True)
+    fs, op, q1, xs, b = _problem(C, (6, 6, 6), 5)
+    for mixed in (1, 2):
+        its = {}
+        for coeffs in ("fp16", "fp32"):
+            x = torch.zeros_like(xs)
+            st = op.solve(q1, b, x, mixed=mixed, rtol=1e-10, pc="mg", mg_coeffs=coeffs)
+            r = torch.zeros_like(xs)
+            op.mat_mult(x, r)
+            relres = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
+            print(f"mixed={mixed} {coeffs}: {st['inner_its']} iterations ({st['outer_its']} outer), relres {relres:.2e}")
+            assert relres <= 1.01e-10
+            its[coeffs] = st["inner_its"]
+        assert abs(its["fp16"] - its["fp32"]) <= 2
 
 
 def _dense_q1(q1):

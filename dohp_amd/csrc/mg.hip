@@ -392,8 +392,21 @@ __global__ void prolong_add_kernel(Box f, Box c, const T *__restrict__ xc, T *__
   xf[F] += acc;
 }
 
-// Ac = P^T Af P, one thread per coarse row (FP64).
-__global__ void rap_kernel(Box f, Box c, const double *__restrict__ Af, double *__restrict__ Ac)
+// 1-D prolongation weight of the coarse node at offset t (in -1..1) from coarse node c at the fine node at offset e
+// (in -2..2) from c's own fine node 2c+1: 1 if e = 2t, 1/2 if |e - 2t| = 1, else 0.
+__host__ __device__ constexpr double pweight(int e, int t)
+{
+  return e == 2 * t ? 1.0 : (e - 2 * t == 1 || e - 2 * t == -1) ? 0.5 : 0.0;
+}
+
+// Ac = P^T Af P, one thread per coarse row (FP64).  Row C gathers over the 27 fine nodes f of its prolongation's
+// support (weight P[f][C]) and their 27 stencil entries A[f][g].  The weights factor by direction: with f at offsets
+// (a, b, d) from C's own fine node and g = f + (si, sj, sk), P[f][C] P[g][C + t] = X(a, si, ti) Y(b, sj, tj)
+// Z(d, sk, tk).  The x part (a, si) is a loop; inside it the 81 (b, d, sj, sk) terms have compile-time y and z weights
+// and are summed into 3 x 3 partial sums in registers (every load unconditional: a support node outside the fine box
+// reads entry 0 with weight 0), which then meet the three x weights.  Entries toward coarse nodes outside the box are
+// zero, as the assembly leaves them on the finest level.
+__global__ void __launch_bounds__(256) rap_kernel(Box f, Box c, const double *__restrict__ Af, double *__restrict__ Ac)
 {
   const long C = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long nc = c.n(), nf = f.n();
@@ -401,38 +414,52 @@ __global__ void rap_kernel(Box f, Box c, const double *__restrict__ Af, double *
   int I, J, K;
   unrank(c, C, I, J, K);
   double acc[27];
-  for (int s = 0; s < 27; s++) acc[s] = 0.0;
+#pragma unroll
+  for (int t = 0; t < 27; t++) acc[t] = 0.0;
   const long sy = f.nz, sx = (long)f.ny * f.nz;
+#pragma unroll 1
   for (int a = -1; a <= 1; a++) {
-    const int fi = 2 * I + 1 + a;
+    const int fi = 2 * I + 1 + a;  // >= 0
     if (fi >= f.nx) continue;
-    for (int b = -1; b <= 1; b++) {
-      const int fj = 2 * J + 1 + b;
-      if (fj >= f.ny) continue;
-      for (int d = -1; d <= 1; d++) {
-        const int fk = 2 * K + 1 + d;
-        if (fk >= f.nz) continue;
-        const double wc = (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0) * (d ? 0.5 : 1.0);
-        const long fr = fi * sx + fj * sy + fk;
-        for (int s = 0; s < 27; s++) {
-          const int gi = fi + s / 9 - 1, gj = fj + (s / 3) % 3 - 1, gk = fk + s % 3 - 1;
-          if (gi < 0 || gi >= f.nx || gj < 0 || gj >= f.ny || gk < 0 || gk >= f.nz) continue;
-          const double afg = Af[s * nf + fr];
-          if (afg == 0.0) continue;
-          int pi[2], pj[2], pk[2];
-          double wi[2], wj[2], wk[2];
-          const int mi = parents(gi, c.nx, pi, wi), mj = parents(gj, c.ny, pj, wj), mk = parents(gk, c.nz, pk, wk);
-          for (int x = 0; x < mi; x++)
-            for (int y = 0; y < mj; y++)
-              for (int z = 0; z < mk; z++) {
-                const int ddi = pi[x] - I, ddj = pj[y] - J, ddk = pk[z] - K;  // in [-1, 1]
-                acc[(ddi + 1) * 9 + (ddj + 1) * 3 + (ddk + 1)] += wc * afg * wi[x] * wj[y] * wk[z];
-              }
+#pragma unroll 1
+    for (int si = -1; si <= 1; si++) {
+      double T[9];
+#pragma unroll
+      for (int u = 0; u < 9; u++) T[u] = 0.0;
+#pragma unroll
+      for (int b = -1; b <= 1; b++)
+#pragma unroll
+        for (int d = -1; d <= 1; d++) {
+          const int fj = 2 * J + 1 + b, fk = 2 * K + 1 + d;
+          const bool in = fj < f.ny && fk < f.nz;
+          const long fr = in ? fi * sx + fj * sy + fk : 0;
+          const double w = in ? pweight(b, 0) * pweight(d, 0) : 0.0;
+#pragma unroll
+          for (int sj = -1; sj <= 1; sj++)
+#pragma unroll
+            for (int sk = -1; sk <= 1; sk++) {
+              const double v = w * Af[((si + 1) * 9 + (sj + 1) * 3 + (sk + 1)) * nf + fr];
+#pragma unroll
+              for (int tj = -1; tj <= 1; tj++)
+#pragma unroll
+                for (int tk = -1; tk <= 1; tk++) {
+                  const double q = pweight(b + sj, tj) * pweight(d + sk, tk);
+                  if (q != 0.0) T[(tj + 1) * 3 + (tk + 1)] = fma(v, q, T[(tj + 1) * 3 + (tk + 1)]);
+                }
+            }
         }
+      const double wa = pweight(a, 0);
+#pragma unroll
+      for (int ti = -1; ti <= 1; ti++) {
+        const double x = wa * pweight(a + si, ti);
+#pragma unroll
+        for (int u = 0; u < 9; u++) acc[(ti + 1) * 9 + u] = fma(x, T[u], acc[(ti + 1) * 9 + u]);
       }
     }
   }
-  for (int s = 0; s < 27; s++) Ac[s * nc + C] = acc[s];
+#pragma unroll
+  for (int t = 0; t < 27; t++)
+    Ac[t * nc + C] = inside(c, I + t / 9 - 1, J + (t / 3) % 3 - 1, K + t % 3 - 1) ? acc[t] : 0.0;
 }
 
 // x = Ainv b, one wavefront per row (lanes stride the row, then a butterfly reduction).

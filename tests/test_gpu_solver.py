@@ -68,8 +68,13 @@ def test_stencil_product_equals_csr(C):
     np.testing.assert_allclose(y.cpu().numpy(), ref, rtol=0, atol=1e-14 * np.abs(ref).max())
 
 
-def test_multigrid_galerkin_levels_and_symmetry(C):
-    fs, op, q1, xs, b = _problem(C, (4, 4, 4), 4)
+@pytest.mark.parametrize("M,deg", [((4, 4, 4), 4), ((3, 4, 5), 3)])
+def test_multigrid_galerkin_levels_and_symmetry(C, M, deg):
+    """Every coarse level is P^T A P of the level above (scipy, with P the Kronecker product of the 1-D linear
+    interpolations), on boxes of odd dimensions (15^3) and of even ones (8 x 11 x 14, whose last fine node of an
+    even dimension has one coarse parent); the V-cycle is symmetric and positive."""
+    print(f"galerkin M={M} p={deg}", flush=True)
+    fs, op, q1, xs, b = _problem(C, M, deg)
     mg = C.MG(q1, smooth_degree=2, max_coarse=64)
     assert mg.nlevels >= 3
     box0, st0 = mg.level(0)

@@ -202,26 +202,51 @@ __global__ void __launch_bounds__(256) half16_setup_kernel(Box bx, const double 
   sc[i] = si;
 }
 
-// (A x)_i on the FP16 coefficients, accumulated in FP32.
-__device__ __forceinline__ float half16_row(const Box &bx, const _Float16 *__restrict__ H, const float *__restrict__ dg,
-                                            const float *__restrict__ sc, const float *__restrict__ x, int I, int J,
-                                            int K, long i)
+// The FP16 products stage the scaled iterate s x of their block's neighbourhood in LDS: planes I-1..I+1, rows
+// J0-1..J0+kBY, columns K0-1..K0+kBX (zero outside the box), loaded once by the block (every load unconditional, at a
+// clamped index) instead of 26 loads of x and 26 of s per row; each row then reads its neighbours from the tile.
+constexpr int kTX = kBX + 2, kTY = kBY + 2, kTile = 3 * kTY * kTX, kTileIt = (kTile + kBX * kBY - 1) / (kBX * kBY);
+__device__ __forceinline__ void stage_scaled(const Box &bx, const float *__restrict__ sc, const float *__restrict__ x,
+                                             float *tile)
+{
+  const int I = blockIdx.z, J0 = blockIdx.y * kBY, K0 = blockIdx.x * kBX;
+  const long sy = bx.nz, sx = (long)bx.ny * bx.nz;
+  const int tid = threadIdx.y * kBX + threadIdx.x;
+  float v[kTileIt];
+#pragma unroll
+  for (int u = 0; u < kTileIt; u++) {
+    const int t = min(tid + u * kBX * kBY, kTile - 1);
+    const int p = t / (kTY * kTX), rem = t - p * (kTY * kTX), jj = rem / kTX, kk = rem - jj * kTX;
+    const int gi = I + p - 1, gj = J0 + jj - 1, gk = K0 + kk - 1;
+    const bool in = inside(bx, gi, gj, gk);
+    const long g = in ? gi * sx + gj * sy + gk : 0;
+    const float y = sc[g] * x[g];
+    v[u] = in ? y : 0.f;
+  }
+#pragma unroll
+  for (int u = 0; u < kTileIt; u++)
+    if (tid + u * kBX * kBY < kTile) tile[tid + u * kBX * kBY] = v[u];
+  __syncthreads();
+}
+
+// sum_j h_ij s_j x_j over the 26 neighbours of row i (the thread's row of the block), from the staged tile.  A
+// neighbour outside the box has a zero tile entry; its coefficient is read at a clamped index.
+__device__ __forceinline__ float half16_tile_row(const Box &bx, const _Float16 *__restrict__ H, const float *tile,
+                                                 long i)
 {
   const long n = bx.n(), sy = bx.nz, sx = (long)bx.ny * bx.nz;
-  // unconditional loads at clamped indices, dropped by selects (see stencil_mult_kernel)
+  const int tj = threadIdx.y + 1, tk = threadIdx.x + 1;
   float acc = 0;
 #pragma unroll
   for (int s = 14; s < 27; s++) {
     const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
     const long off = di * sx + dj * sy + dk;
-    const bool up = inside(bx, I + di, J + dj, K + dk), lo = inside(bx, I - di, J - dj, K - dk);
-    const long iu = up ? i + off : i, il = lo ? i - off : i;
-    const float vu = (float)H[(s - 14) * n + i] * (sc[iu] * x[iu]);
-    const float vl = (float)H[(s - 14) * n + il] * (sc[il] * x[il]);
-    acc += up ? vu : 0.f;
-    acc += lo ? vl : 0.f;
+    const long il = i - off >= 0 && i - off < n ? i - off : i;
+    const float hu = (float)H[(s - 14) * n + i], hl = (float)H[(s - 14) * n + il];
+    acc += hu * tile[((1 + di) * kTY + tj + dj) * kTX + tk + dk];
+    acc += hl * tile[((1 - di) * kTY + tj - dj) * kTX + tk - dk];
   }
-  return dg[i] * x[i] + sc[i] * acc;
+  return acc;
 }
 
 __global__ void __launch_bounds__(256) half16_residual_kernel(Box bx, const _Float16 *__restrict__ H,
@@ -229,10 +254,13 @@ __global__ void __launch_bounds__(256) half16_residual_kernel(Box bx, const _Flo
                                                               const float *__restrict__ b, const float *__restrict__ x,
                                                               float *__restrict__ r)
 {
+  __shared__ float tile[kTile];
+  stage_scaled(bx, sc, x, tile);
   int I, J, K;
   long i;
   if (!box_index(bx, I, J, K, i)) return;
-  r[i] = (b ? b[i] : 0.f) - half16_row(bx, H, dg, sc, x, I, J, K, i);
+  const float ax = dg[i] * x[i] + sc[i] * half16_tile_row(bx, H, tile, i);
+  r[i] = (b ? b[i] : 0.f) - ax;
 }
 
 // One Chebyshev step of the smoother on the FP16 coefficients, out of place (Jacobi needs the old iterate at the
@@ -243,13 +271,16 @@ __global__ void __launch_bounds__(256) half16_cheb_kernel(Box bx, const _Float16
                                                           float *__restrict__ xout, float *__restrict__ d, float a,
                                                           float c, int keep_d)
 {
+  __shared__ float tile[kTile];
+  stage_scaled(bx, sc, xin, tile);
   int I, J, K;
   long i;
   if (!box_index(bx, I, J, K, i)) return;
-  const float r = b[i] - half16_row(bx, H, dg, sc, xin, I, J, K, i);
+  const float xi = xin[i];
+  const float r = b[i] - (dg[i] * xi + sc[i] * half16_tile_row(bx, H, tile, i));
   const float dn = (a != 0.f ? a * d[i] : 0.f) + c * r / dg[i];
   if (keep_d) d[i] = dn;
-  xout[i] = xin[i] + dn;
+  xout[i] = xi + dn;
 }
 
 // First step from a zero iterate: x = d = c b / d_ii.

@@ -86,3 +86,22 @@ def test_error_reporting_without_gpu():
     assert "basis size" in str(ei.value)
     with pytest.raises(C.DohpError):
         C.tensor_rule(7, 0, 4)
+
+
+def test_failed_device_allocation_is_an_error_not_a_crash():
+    """Without a device every allocation fails: dFSCreateBlock returns dohpERR_OOM naming the size it asked for, and a
+    second call asks again (a buffer whose allocation failed keeps no size, so a retry never skips the allocation and
+    writes through a null pointer)."""
+    C = pytest.importorskip("dohp_amd.capi")
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("exercises the allocation failure of a host without a GPU")
+    from dohp_amd.mesh import block_vertices
+    V = block_vertices((2, 2, 2))
+    msgs = []
+    for _ in range(2):
+        with pytest.raises(C.DohpError) as ei:
+            C.FS((2, 2, 2), 3, V)
+        assert ei.value.code == C.ERR_OOM
+        msgs.append(str(ei.value))
+    assert "hipMalloc failed for" in msgs[0] and msgs[0] == msgs[1]

@@ -527,6 +527,36 @@ __global__ void sumsq_kernel(long n, const double *__restrict__ a, double *__res
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// Start vector of the power iteration: a counter-based hash (splitmix64) of the row, uniform in [0.25, 1.25).
+__global__ void power_start_kernel(long n, double *__restrict__ v)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = (uint64_t)i + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  v[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0) + 0.25;
+}
+
+// v = w / ||w||, with ||w||^2 the sum of sumsq_kernel's 256 partial sums taken in order (the same sum on every
+// block, no host round trip); block 0 also stores ||w|| (norm may be null).  v may be w.
+__global__ void normalize_kernel(long n, const double *__restrict__ part, const double *w, double *v,
+                                 double *__restrict__ norm)
+{
+  __shared__ double inv;
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int k = 0; k < 256; k++) t += part[k];
+    const double nw = sqrt(t);
+    inv = 1.0 / nw;
+    if (blockIdx.x == 0 && norm) *norm = nw;
+  }
+  __syncthreads();
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = inv * w[i];
+}
+
 __global__ void scale_kernel(long n, double s, const double *__restrict__ dinv, const double *__restrict__ a,
                              double *__restrict__ b)
 {
@@ -615,36 +645,22 @@ dErr estimate_lmax(MGLevel &lv, hipStream_t st, int its)
   DevBuf<double> v, w, part;
   DOHP_CALL(v.alloc(lv.n));
   DOHP_CALL(w.alloc(lv.n));
-  DOHP_CALL(part.alloc(256));
-  std::vector<double> h(lv.n);
-  uint64_t s = 0x2545F4914F6CDD1DULL;
-  for (long i = 0; i < lv.n; i++) {
-    s ^= s << 13;
-    s ^= s >> 7;
-    s ^= s << 17;
-    h[i] = (double)(s >> 11) / 9007199254740992.0 + 0.25;
-  }
-  DOHP_HIP(hipMemcpyAsync(v.ptr, h.data(), sizeof(double) * lv.n, hipMemcpyHostToDevice, st));
-  auto norm = [&](const double *a, double *out) -> dErr {
-    hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, st, lv.n, a, part.ptr);
-    double p[256];
-    DOHP_HIP(hipMemcpyAsync(p, part.ptr, sizeof(p), hipMemcpyDeviceToHost, st));
-    DOHP_HIP(hipStreamSynchronize(st));
-    double t = 0;
-    for (int i = 0; i < 256; i++) t += p[i];
-    *out = std::sqrt(t);
-    return 0;
-  };
-  double nv, nw = 0;
-  DOHP_CALL(norm(v.ptr, &nv));
-  hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0 / nv, nullptr, v.ptr, v.ptr);
+  DOHP_CALL(part.alloc(257));  // 256 partial sums and the last norm
+  hipLaunchKernelGGL(power_start_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, v.ptr);
+  // the iterations stay on the device: one read of the last norm at the end
+  hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, st, lv.n, v.ptr, part.ptr);
+  hipLaunchKernelGGL(normalize_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, part.ptr, v.ptr, v.ptr, nullptr);
   for (int k = 0; k < its; k++) {
     residual<double>(lv.box, lv.H64.ptr, nullptr, v.ptr, w.ptr, st);  // w = -A v
     hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, -1.0, lv.dinv64.ptr, w.ptr, w.ptr);
-    DOHP_CALL(norm(w.ptr, &nw));
-    hipLaunchKernelGGL(scale_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, 1.0 / nw, nullptr, w.ptr, v.ptr);
+    hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, st, lv.n, w.ptr, part.ptr);
+    hipLaunchKernelGGL(normalize_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, part.ptr, w.ptr, v.ptr,
+                       part.ptr + 256);
   }
   DOHP_HIP(hipGetLastError());
+  double nw = 0;
+  if (its > 0) DOHP_HIP(hipMemcpyAsync(&nw, part.ptr + 256, sizeof(double), hipMemcpyDeviceToHost, st));
+  DOHP_HIP(hipStreamSynchronize(st));
   lv.lmax = nw;
   return 0;
 }

@@ -265,11 +265,11 @@ class ScaledProblem:
         self.op.set_q1_cell_scales(None)
         return self.q1s
 
-    def energy_gradient(self, gsig):
-        """Mean relative energy error of the learned V-cycle over the pairs (shared optimal scale) and its gradient
-        in the sub-cell scales added into gsig."""
+    def energy_gradient(self, gsig, smooth_degree=2):
+        """Mean relative energy error of the learned V-cycle (Chebyshev smoothing of the given degree) over the pairs
+        (shared optimal scale) and its gradient in the sub-cell scales added into gsig."""
         torch, C = self.torch, self.C
-        mg = C.MG(self.q1s, smooth_degree=2)
+        mg = C.MG(self.q1s, smooth_degree=smooth_degree)
         zs, terms = [], []
         for (r, e) in self.pairs:
             z = torch.zeros_like(r)
@@ -307,13 +307,13 @@ def torch_zeros(torch, like):
 
 
 def fit_q1_scales(C, torch, train_seeds, val_seeds, M=(12, 12, 12), deg=6, epochs=30, lr=3e-3, seed=0, npairs=8,
-                  every=3, nrandom=2, val_M=None, scale_penalty=0.0):
+                  every=3, nrandom=2, val_M=None, scale_penalty=0.0, train_degree=1):
     """Train the sub-cell scale network on residual/error pairs of plain solves of the training problems (one Adam
     step per problem and epoch on the mean energy error of its pairs); keep the parameters with the fewest CG
     iterations on the validation problems (of size val_M, default M).  scale_penalty adds
     0.5 * scale_penalty * (mean log sigma)^2: the loss does not see a common factor of all scales (alpha absorbs it),
-    and the penalty keeps that factor near 1 instead of drifting to the clamp.  Returns (net or None if no epoch beat
-    the plain matrix, report)."""
+    and the penalty keeps that factor near 1 instead of drifting to the clamp.  train_degree: smoothing degree of the
+    V-cycle inside the loss (1, the degree the learned matrix is solved with).  Returns (net or None if no epoch beat the plain matrix, report)."""
     train = [ScaledProblem(C, torch, M, deg, sd, npairs=npairs, every=every, nrandom=nrandom) for sd in train_seeds]
     val = [ScaledProblem(C, torch, val_M or M, deg, sd) for sd in val_seeds]
     plain_its = [v.cg_its() for v in val]
@@ -325,7 +325,7 @@ def fit_q1_scales(C, torch, train_seeds, val_seeds, M=(12, 12, 12), deg=6, epoch
         for p in train:
             p.assemble(net)
             gsig = torch.zeros((p.ncell, 3), dtype=torch.float32, device="cuda")
-            Es.append(p.energy_gradient(gsig))
+            Es.append(p.energy_gradient(gsig, train_degree))
             if scale_penalty:
                 ls = torch.log(p.sig)
                 gsig += scale_penalty * float(ls.mean()) / ls.numel() / p.sig

@@ -307,15 +307,17 @@ def torch_zeros(torch, like):
 
 
 def fit_q1_scales(C, torch, train_seeds, val_seeds, M=(12, 12, 12), deg=6, epochs=30, lr=3e-3, seed=0, npairs=8,
-                  every=3, nrandom=2, val_M=None, scale_penalty=0.0, train_degree=1):
+                  every=3, nrandom=2, val_M=None, scale_penalty=0.0, train_degree=1, q1_mode=0):
     """Train the sub-cell scale network on residual/error pairs of plain solves of the training problems (one Adam
     step per problem and epoch on the mean energy error of its pairs); keep the parameters with the fewest CG
     iterations on the validation problems (of size val_M, default M).  scale_penalty adds
     0.5 * scale_penalty * (mean log sigma)^2: the loss does not see a common factor of all scales (alpha absorbs it),
     and the penalty keeps that factor near 1 instead of drifting to the clamp.  train_degree: smoothing degree of the
-    V-cycle inside the loss (1, the degree the learned matrix is solved with).  Returns (net or None if no epoch beat the plain matrix, report)."""
-    train = [ScaledProblem(C, torch, M, deg, sd, npairs=npairs, every=every, nrandom=nrandom) for sd in train_seeds]
-    val = [ScaledProblem(C, torch, val_M or M, deg, sd) for sd in val_seeds]
+    V-cycle inside the loss (1, the degree the learned matrix is solved with).  q1_mode 1 learns scales on top of the
+    corrected embedded-Q1 arithmetic (and compares with the corrected matrix).  Returns (net or None if no epoch beat the plain matrix, report)."""
+    train = [ScaledProblem(C, torch, M, deg, sd, q1_mode=q1_mode, npairs=npairs, every=every, nrandom=nrandom)
+             for sd in train_seeds]
+    val = [ScaledProblem(C, torch, val_M or M, deg, sd, q1_mode=q1_mode) for sd in val_seeds]
     plain_its = [v.cg_its() for v in val]
     net = C.CellMLP(cell_init_params(C.CellMLP.nparam, seed))
     best = (sum(plain_its), None, -1)

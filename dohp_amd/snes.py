@@ -182,13 +182,13 @@ def _line_search_bt(op, x, f, fnorm, Y, t, alpha=1e-4, minlambda=1e-12):
 
 
 def newton(op, x, pc=None, rtol=1e-10, atol=1e-50, stol=1e-8, max_it=50, linesearch="basic", ksp_rtol=1e-5,
-           ksp_atol=1e-50, ksp_restart=30, ksp_max_it=10000, ksp_type="gmres"):
+           ksp_atol=1e-50, ksp_restart=30, ksp_max_it=10000, ksp_type="gmres", mg_degree=2, mg_fp32=False):
     """Solve F(x) = 0 for an operator with op.function(x, f) (the residual, which also records x as the
     linearization state) and op.mat_mult(v, y) (the Jacobian action at that state).  x: the initial guess
     on the owned nodes (updated in place).  pc: None (unpreconditioned), "mg" (multigrid V-cycle on the
-    embedded-Q1 matrix assembled at each iterate; op.assemble_q1) or a callable pc(op, x) returning
-    M(v, out) for the current iterate; ksp_type "gmres" or "cg".  Returns (x, ||F|| history, KSP histories,
-    reason)."""
+    embedded-Q1 matrix assembled at each iterate; op.assemble_q1; Chebyshev smoothing of degree mg_degree, applied
+    in FP32 on FP16 coefficients with mg_fp32) or a callable pc(op, x) returning M(v, out) for the current iterate;
+    ksp_type "gmres" or "cg".  Returns (x, ||F|| history, KSP histories, reason)."""
     f = torch.empty_like(x)
     t = torch.empty_like(x)
     op.function(x, f)
@@ -204,8 +204,17 @@ def newton(op, x, pc=None, rtol=1e-10, atol=1e-50, stol=1e-8, max_it=50, linesea
         elif pc == "mg":
             from dohp_amd.capi import MG
             q1 = op.assemble_q1(x, mat=q1)
-            mg = MG(q1)
-            M = mg.apply
+            mg = MG(q1, smooth_degree=mg_degree)
+            if mg_fp32:
+                r32, z32 = torch.empty(x.shape, dtype=torch.float32, device=x.device), \
+                    torch.empty(x.shape, dtype=torch.float32, device=x.device)
+
+                def M(v, out, mg=mg, r32=r32, z32=z32):
+                    r32.copy_(v)
+                    mg.apply_f32(r32, z32)
+                    out.copy_(z32)
+            else:
+                M = mg.apply
         else:
             M = pc(op, x)
         if ksp_type == "cg":

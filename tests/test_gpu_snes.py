@@ -81,12 +81,15 @@ def test_newton_gmres_histories_match_the_reference(C, name):
         match(kh, kr, kr[0])
 
 
-def test_newton_with_multigrid_preconditioner(C):
+@pytest.mark.parametrize("q1_mode,mg_degree,mg_fp32", [(0, 2, False), (0, 2, True), (1, 1, True)])
+def test_newton_with_multigrid_preconditioner(C, q1_mode, mg_degree, mg_fp32):
     """The device-only configuration: Newton (backtracking) with GMRES preconditioned by the multigrid V-cycle on
-    the embedded-Q1 matrix of each iterate, on a perturbed 6^3 mesh at p = 4 (the line search backtracks in the
-    first steps from u = 0); converges to the SNES tolerance, and the discrete solution agrees with the
-    manufactured one to discretization accuracy at the nodes."""
+    the embedded-Q1 matrix of each iterate (reference or corrected arithmetic; FP64, or FP32 on FP16 coefficients),
+    on a perturbed 6^3 mesh at p = 4 (the line search backtracks in the first steps from u = 0); converges to the
+    SNES tolerance, and the discrete solution agrees with the manufactured one to discretization accuracy at the
+    nodes."""
     from dohp_amd import snes
+    print(f"newton q1_mode={q1_mode} mg_degree={mg_degree} mg_fp32={mg_fp32}", flush=True)
     M, bdeg = (6, 6, 6), 4
     V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=3)
     fs = C.FS(M, bdeg, V)
@@ -95,8 +98,9 @@ def test_newton_with_multigrid_preconditioner(C):
     S = O.BlockSpace(M, bdeg, V)
     E = O.Ellip(S, **prm)
     fs.dirichlet_commit(E.dirichlet_values())
+    op.set_q1_mode(q1_mode)
     x = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
-    x, hist, ksp, reason = snes.newton(op, x, pc="mg", linesearch="bt")
+    x, hist, ksp, reason = snes.newton(op, x, pc="mg", linesearch="bt", mg_degree=mg_degree, mg_fp32=mg_fp32)
     print("snes", ["%.3g" % h for h in hist], "ksp its", [len(k) - 1 for k in ksp])
     assert reason > 0 and hist[-1] <= 1e-10 * hist[0]
     exact = E.exact_solution(S.nodal_coordinates()[:fs.nowned])

@@ -152,6 +152,7 @@ struct _p_dohpMat {
   int mg_degree = 0;
   int corrected = 0;  // assembled with the corrected sub-cell arithmetic (embedded-Q1 mode 1)
   int scaled = 0;     // assembled with learned per-sub-cell derivative scales
+  bool csr_stale = false;  // block-space matrix: CSR values and FP32 stencil not yet derived from the stencil form
   dohp::DevBuf<double> q1tab;  // the assembly kernel's 1-D tables
   ~_p_dohpMat() { dohpMGDestroy(&mg); }
 };
@@ -159,6 +160,8 @@ struct _p_dohpMat {
 dErr dohpMatPrepareStencil_Private(_p_dohpMat *A, dohpFS fs, bool *box, hipStream_t st);
 // CSR values (FP64, FP32) and the FP32 stencil from the FP64 stencil form.
 dErr dohpMatStencilToCSR_Private(_p_dohpMat *A, hipStream_t st);
+// Derives the CSR values and FP32 stencil of a block-space matrix on their first use after an assembly.
+dErr dohpMatSyncCSR_Private(_p_dohpMat *A, hipStream_t st);
 dErr dohpStencilMult_Private(const _p_dohpMat *A, const double *x, double *y, hipStream_t st);
 dErr dohpStencilMultF32_Private(const _p_dohpMat *A, const float *x, float *y, hipStream_t st);
 

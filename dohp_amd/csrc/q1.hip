@@ -706,7 +706,10 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   }
   DOHP_HIP(hipGetLastError());
   }
-  if (box) return dohpMatStencilToCSR_Private(A, st);
+  if (box) {  // the multigrid and the FP64 product read the stencil form; the CSR values follow on demand
+    A->csr_stale = true;
+    return 0;
+  }
   hipLaunchKernelGGL(to_f32_kernel, dim3((A->nnz + 255) / 256), dim3(256), 0, st, A->val.ptr, A->val32.ptr, A->nnz);
   DOHP_HIP(hipGetLastError());
   return 0;
@@ -720,9 +723,19 @@ extern "C" dErr dohpMatGetSizes(dohpMat A, long *nrows, long *nnz)
   return 0;
 }
 
+dErr dohpMatSyncCSR_Private(_p_dohpMat *A, hipStream_t st)
+{
+  if (!A->csr_stale) return 0;
+  DOHP_CALL(dohpMatStencilToCSR_Private(A, st));
+  A->csr_stale = false;
+  return 0;
+}
+
 extern "C" dErr dohpMatGetCSR(dohpMat A, int *rowptr, int *col, double *val)
 {
   DOHP_CHECK(A, dohpERR_ARG, "null matrix");
+  DOHP_HIP(hipDeviceSynchronize());
+  DOHP_CALL(dohpMatSyncCSR_Private(A, nullptr));
   DOHP_HIP(hipDeviceSynchronize());
   if (rowptr) DOHP_HIP(hipMemcpy(rowptr, A->rowptr.ptr, sizeof(int) * (A->nrows + 1), hipMemcpyDeviceToHost));
   if (col) DOHP_HIP(hipMemcpy(col, A->col.ptr, sizeof(int) * A->nnz, hipMemcpyDeviceToHost));
@@ -743,7 +756,10 @@ extern "C" dErr dohpMatMult(dohpMat A, const double *x, double *y, void *stream)
 extern "C" dErr dohpMatMultF32(dohpMat A, const float *x, float *y, void *stream)
 {
   DOHP_CHECK(A && x && y, dohpERR_ARG, "null argument");
-  if (A->st32.ptr) return dohpStencilMultF32_Private(A, x, y, (hipStream_t)stream);
+  if (A->st32.ptr) {
+    DOHP_CALL(dohpMatSyncCSR_Private(A, (hipStream_t)stream));
+    return dohpStencilMultF32_Private(A, x, y, (hipStream_t)stream);
+  }
   hipLaunchKernelGGL(csr_spmv_kernel<float>, dim3((A->nrows + 255) / 256), dim3(256), 0, (hipStream_t)stream, A->nrows,
                      A->rowptr.ptr, A->col.ptr, A->val32.ptr, x, y);
   DOHP_HIP(hipGetLastError());

@@ -150,8 +150,11 @@ __global__ void __launch_bounds__(128) q1_cell_kernel(int nelem, const int *__re
     double PU[8], PG0[8], PG1[8], PG2[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) PU[q] = PG0[q] = PG1[q] = PG2[q] = 0.0;
+    // the state enters the block only through eta(grad u) and s (p != 2) and lambda e^u: for the linear operator
+    // (p = 2, lambda = 0) eta = 1 and s = 0 whatever the state, and its interpolation is skipped
+    const int na = ustate && (ph.p != 2.0 || ph.lambda != 0.0) ? N : 0;
 #pragma unroll 1
-    for (int a = 0; a < N; a++) {
+    for (int a = 0; a < na; a++) {
       double Y[2][2][3];  // [jj][kk][BB | DB | BD]
 #pragma unroll
       for (int jj = 0; jj < 2; jj++)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -102,6 +103,72 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// Device buffers kept between calls of one object: hipFree waits for the device (about 0.15 ms a call), so the work
+// vectors of a solve are taken from the operator's pool and given back to it, not freed.  A buffer is reused only at
+// the same size; when an allocation fails, the idle buffers are freed and it is tried once more.  Not thread-safe
+// (one caller per object, like the reference's objects).
+struct Pool {
+  std::multimap<size_t, void *> idle;
+  Pool() = default;
+  Pool(const Pool &) = delete;
+  Pool &operator=(const Pool &) = delete;
+  void trim()
+  {
+    for (auto &kv : idle) (void)hipFree(kv.second);
+    idle.clear();
+  }
+  ~Pool() { trim(); }
+  void *take(size_t bytes)
+  {
+    auto it = idle.find(bytes);
+    if (it != idle.end()) {
+      void *p = it->second;
+      idle.erase(it);
+      return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) == hipSuccess) return p;
+    (void)hipGetLastError();
+    trim();
+    if (hipMalloc(&p, bytes) == hipSuccess) return p;
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  void give(size_t bytes, void *p) { idle.emplace(bytes, p); }
+};
+
+// A work vector from a Pool, given back when it goes out of scope (after the work queued on it in stream order).
+template <typename T>
+struct PoolBuf {
+  Pool *pool = nullptr;
+  T *ptr = nullptr;
+  size_t n = 0;
+  PoolBuf() = default;
+  PoolBuf(const PoolBuf &) = delete;
+  PoolBuf &operator=(const PoolBuf &) = delete;
+  dErr alloc(Pool &p, size_t count)
+  {
+    release();
+    if (count == 0) return 0;
+    void *q = p.take(count * sizeof(T));
+    if (!q) {
+      set_error("hipMalloc failed for " + std::to_string(count * sizeof(T)) + " bytes");
+      return dohpERR_OOM;
+    }
+    pool = &p;
+    ptr = (T *)q;
+    n = count;
+    return 0;
+  }
+  void release()
+  {
+    if (ptr) pool->give(n * sizeof(T), ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  ~PoolBuf() { release(); }
+};
+
 // Current device and its compute-unit count (the properties are read once per device).
 inline dErr current_device_cus(int &dev, int &cus)
 {
@@ -132,6 +199,8 @@ extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hi
 // y = J x without the entry points' non-finite check (no wait for the stream): the library's solvers.
 dErr dohpEllipMatMult_Private(dohpEllip op, const double *x, double *y, hipStream_t st);
 dErr dohpEllipMatMult_Private(dohpEllip op, const float *x, float *y, hipStream_t st);
+// The operator's pool of solver work vectors.
+dohp::Pool *dohpEllipPool_Private(dohpEllip op);
 
 // Sparse matrix on the owned nodes.  CSR is the assembled form; on a block space the owned nodes
 // form a box of the node lattice (nx * ny * nz, lexicographic, last index fastest) and every row

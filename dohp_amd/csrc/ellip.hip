@@ -367,6 +367,7 @@ __global__ void lattice_state_kernel(const double *__restrict__ u, const double 
 }  // namespace
 
 struct _p_dohpEllip {
+  dohp::Pool pool;  // work vectors of the solves on this operator (freed with it)
   dohpFS fs = nullptr;
   dohpEllipParam prm{};
   int scatter = 1;  // 0 = colours (deterministic), 1 = atomics
@@ -835,6 +836,8 @@ extern "C" dErr dohpEllipFunctionState_Private(dohpEllip op, const double *u, hi
   DOHP_HIP(hipGetLastError());
   return 0;
 }
+
+dohp::Pool *dohpEllipPool_Private(dohpEllip op) { return &op->pool; }
 
 extern "C" dErr dohpEllipGetQ1Context_Private(dohpEllip op, dohpFS *fs, dohpEllipParam *prm, const double **kvert,
                                               const double **state, int *q1mode)

@@ -111,7 +111,8 @@ inline dim3 grid(long n) { return dim3((n + 255) / 256); }
 struct Ctx {
   hipStream_t st;
   long n;
-  DevBuf<double> part, res;
+  Pool *pool;  // the operator's: every work vector of a solve comes from it
+  PoolBuf<double> part, res;
 };
 
 template <typename T>
@@ -153,7 +154,7 @@ struct Cheb {
   const T *dinv;
   double lmin, lmax;
   int degree;
-  DevBuf<T> res, d, w;
+  PoolBuf<T> res, d, w;
   dErr apply(Ctx &c, const T *r, T *z)
   {
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
@@ -239,9 +240,9 @@ struct Budget {
 // Largest eigenvalue of D^{-1} A_q1 by power iteration (FP64).
 dErr estimate_lmax(Ctx &c, dohpMat q1, const double *dinv, int its, double *lmax)
 {
-  DevBuf<double> v, w;
-  DOHP_CALL(v.alloc(c.n));
-  DOHP_CALL(w.alloc(c.n));
+  PoolBuf<double> v, w;
+  DOHP_CALL(v.alloc(*c.pool, c.n));
+  DOHP_CALL(w.alloc(*c.pool, c.n));
   std::vector<double> h(c.n);
   uint64_t s = 0x9E3779B97F4A7C15ULL;
   for (long i = 0; i < c.n; i++) {
@@ -272,11 +273,11 @@ template <typename T>
 dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double rtol, int max_it, int *its,
          double *rnorm, double *bnorm, T *snap = nullptr, int nsnap = 0, int every = 0, int *nsnapped = nullptr)
 {
-  DevBuf<T> r, z, p, q;
-  DOHP_CALL(r.alloc(c.n));
-  DOHP_CALL(z.alloc(c.n));
-  DOHP_CALL(p.alloc(c.n));
-  DOHP_CALL(q.alloc(c.n));
+  PoolBuf<T> r, z, p, q;
+  DOHP_CALL(r.alloc(*c.pool, c.n));
+  DOHP_CALL(z.alloc(*c.pool, c.n));
+  DOHP_CALL(p.alloc(*c.pool, c.n));
+  DOHP_CALL(q.alloc(*c.pool, c.n));
   DOHP_CALL(Ops<T>::A(op, x, q.ptr, c.st));
   DOHP_HIP(hipMemcpyAsync(r.ptr, b, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
   DOHP_CALL(axpby<T>(c, T(-1), q.ptr, T(1), r.ptr));
@@ -368,16 +369,17 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
   long nrows;
   DOHP_CALL(dohpMatGetSizes(q1, &nrows, nullptr));
   c.n = nrows;
-  DOHP_CALL(c.part.alloc(kRedBlocks));
-  DOHP_CALL(c.res.alloc(1));
+  c.pool = dohpEllipPool_Private(op);
+  DOHP_CALL(c.part.alloc(*c.pool, kRedBlocks));
+  DOHP_CALL(c.res.alloc(*c.pool, 1));
   *stats = dohpSolveStats{};
   // Jacobi scaling and, for the one-level preconditioner, the Chebyshev interval
   // [lmax/ratio, 1.05 lmax] of D^{-1} A_q1
-  DevBuf<double> diag, dinv;
-  DevBuf<float> dinv32;
-  DOHP_CALL(diag.alloc(c.n));
-  DOHP_CALL(dinv.alloc(c.n));
-  DOHP_CALL(dinv32.alloc(c.n));
+  PoolBuf<double> diag, dinv;
+  PoolBuf<float> dinv32;
+  DOHP_CALL(diag.alloc(*c.pool, c.n));
+  DOHP_CALL(dinv.alloc(*c.pool, c.n));
+  DOHP_CALL(dinv32.alloc(*c.pool, c.n));
   DOHP_CALL(dohpMatGetDiagonal(q1, diag.ptr, c.st));
   hipLaunchKernelGGL(inv_kernel, grid(c.n), dim3(256), 0, c.st, c.n, diag.ptr, dinv.ptr, dinv32.ptr);
   double lmax = 1.0;
@@ -404,9 +406,9 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
   Budget bud;
   bud.fallback_its = mlp || q1fb ? opts->fallback_its : 0;
   // learned node weights (computed once: the features do not depend on the residual)
-  DevBuf<double> wts;
-  DevBuf<double> tmp64;
-  DevBuf<float> tmp32;
+  PoolBuf<double> wts;
+  PoolBuf<double> tmp64;
+  PoolBuf<float> tmp32;
   if (mlp) {
     dohpFS fs;
     dohpEllipParam prm;
@@ -414,22 +416,22 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kv, &stt, nullptr));
     long np;
     DOHP_CALL(dohpMLPNumPatches(q1, &np));
-    DevBuf<float> X;
-    DOHP_CALL(X.alloc((size_t)(np > 0 ? np : 1) * dohpMLPNumInputs()));
-    DOHP_CALL(wts.alloc(c.n));
+    PoolBuf<float> X;
+    DOHP_CALL(X.alloc(*c.pool, (size_t)(np > 0 ? np : 1) * dohpMLPNumInputs()));
+    DOHP_CALL(wts.alloc(*c.pool, c.n));
     DOHP_CALL(dohpMLPFeatures(q1, fs, dinv.ptr, X.ptr, c.st));
     DOHP_CALL(dohpMLPWeights(mlp, q1, X.ptr, wts.ptr, nullptr, c.st));
-    DOHP_CALL(tmp64.alloc(c.n));
-    DOHP_CALL(tmp32.alloc(c.n));
+    DOHP_CALL(tmp64.alloc(*c.pool, c.n));
+    DOHP_CALL(tmp32.alloc(*c.pool, c.n));
     DOHP_HIP(hipStreamSynchronize(c.st));
   }
   const auto t_setup = std::chrono::steady_clock::now();
   stats->setup_seconds = std::chrono::duration<double>(t_setup - t0).count();
   if (opts->mixed == 2) {  // FP64 PCG, preconditioner in FP32
     Cheb<float> C32{q1, dinv32.ptr, lmin, lmax, degree};
-    DOHP_CALL(C32.res.alloc(c.n));
-    DOHP_CALL(C32.d.alloc(c.n));
-    DOHP_CALL(C32.w.alloc(c.n));
+    DOHP_CALL(C32.res.alloc(*c.pool, c.n));
+    DOHP_CALL(C32.d.alloc(*c.pool, c.n));
+    DOHP_CALL(C32.w.alloc(*c.pool, c.n));
     Prec<float> M32;
     M32.cheb = &C32;
     M32.mg = mgh.mg;
@@ -437,9 +439,9 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     M32.s = wts.ptr;
     M32.tmp = tmp32.ptr;
     M32.learned_on = mlp != nullptr || q1fb != nullptr;
-    DevBuf<float> lr, lz;
-    DOHP_CALL(lr.alloc(c.n));
-    DOHP_CALL(lz.alloc(c.n));
+    PoolBuf<float> lr, lz;
+    DOHP_CALL(lr.alloc(*c.pool, c.n));
+    DOHP_CALL(lz.alloc(*c.pool, c.n));
     Prec<double> M;
     M.low = &M32;
     M.lr = lr.ptr;
@@ -453,9 +455,9 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     stats->bnorm = bn;
   } else if (!opts->mixed) {
     Cheb<double> C64{q1, dinv.ptr, lmin, lmax, degree};
-    DOHP_CALL(C64.res.alloc(c.n));
-    DOHP_CALL(C64.d.alloc(c.n));
-    DOHP_CALL(C64.w.alloc(c.n));
+    DOHP_CALL(C64.res.alloc(*c.pool, c.n));
+    DOHP_CALL(C64.d.alloc(*c.pool, c.n));
+    DOHP_CALL(C64.w.alloc(*c.pool, c.n));
     Prec<double> M;
     M.cheb = &C64;
     M.mg = mgh.mg;
@@ -472,9 +474,9 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     stats->bnorm = bn;
   } else {
     Cheb<float> C32{q1, dinv32.ptr, lmin, lmax, degree};
-    DOHP_CALL(C32.res.alloc(c.n));
-    DOHP_CALL(C32.d.alloc(c.n));
-    DOHP_CALL(C32.w.alloc(c.n));
+    DOHP_CALL(C32.res.alloc(*c.pool, c.n));
+    DOHP_CALL(C32.d.alloc(*c.pool, c.n));
+    DOHP_CALL(C32.w.alloc(*c.pool, c.n));
     Prec<float> M;
     M.cheb = &C32;
     M.mg = mgh.mg;
@@ -482,12 +484,12 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     M.s = wts.ptr;
     M.tmp = tmp32.ptr;
     M.learned_on = mlp != nullptr || q1fb != nullptr;
-    DevBuf<double> r, t;
-    DevBuf<float> r32, e32;
-    DOHP_CALL(r.alloc(c.n));
-    DOHP_CALL(t.alloc(c.n));
-    DOHP_CALL(r32.alloc(c.n));
-    DOHP_CALL(e32.alloc(c.n));
+    PoolBuf<double> r, t;
+    PoolBuf<float> r32, e32;
+    DOHP_CALL(r.alloc(*c.pool, c.n));
+    DOHP_CALL(t.alloc(*c.pool, c.n));
+    DOHP_CALL(r32.alloc(*c.pool, c.n));
+    DOHP_CALL(e32.alloc(*c.pool, c.n));
     double bb;
     DOHP_CALL(dot(c, b, b, &bb));
     stats->bnorm = std::sqrt(bb);

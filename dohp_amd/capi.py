@@ -102,6 +102,13 @@ class MG:
         check(_lib.dohpMGGetLevel(self.h, int(l), box, st.ctypes.data_as(ctypes.c_void_p)))
         return tuple(box), st.reshape(27, n)
 
+    def refresh(self, q1, stream=None):
+        """Recompute every level from q1 refilled on the same box (no allocation)."""
+        check(_lib.dohpMGRefresh(self.h, q1.h, _stream(stream)))
+        nl, nc, lm = ctypes.c_int(), ctypes.c_long(), ctypes.c_double()
+        check(_lib.dohpMGGetInfo(self.h, ctypes.byref(nl), ctypes.byref(nc), ctypes.byref(lm)))
+        self.lmax = lm.value
+
     def apply(self, r, z, stream=None):
         check(_lib.dohpMGApply(self.h, _ptr(r), _ptr(z), _stream(stream)))
 
@@ -142,6 +149,7 @@ _proto("dohpMGCreate", _vp, ctypes.c_int, ctypes.c_int, _vp, _vp)
 _proto("dohpMGGetInfo", _vp, _vp, _vp, _vp)
 _proto("dohpMGGetLevel", _vp, ctypes.c_int, _vp, _vp)
 _proto("dohpMGApply", _vp, _vp, _vp, _vp)
+_proto("dohpMGRefresh", _vp, _vp, _vp)
 _proto("dohpMGApplyF32", _vp, _vp, _vp, _vp)
 _proto("dohpMGResidual", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)
 _proto("dohpMGResidualF32", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)

@@ -90,8 +90,13 @@ struct DevBuf {
   }
   // Holds exactly `count` elements (reallocated otherwise).
   bool holds(size_t count) const { return ptr != nullptr && n == count; }
+  // Reuses the buffer when it holds exactly count elements, after the device is idle (as freeing it would wait):
+  // work still reading the old contents finishes first.
   dErr upload(const T *h, size_t count) {
-    DOHP_CALL(alloc(count));
+    if (holds(count))
+      DOHP_HIP(hipDeviceSynchronize());
+    else
+      DOHP_CALL(alloc(count));
     if (count) DOHP_HIP(hipMemcpy(ptr, h, count * sizeof(T), hipMemcpyHostToDevice));
     return 0;
   }
@@ -222,6 +227,7 @@ struct _p_dohpMat {
   int corrected = 0;  // assembled with the corrected sub-cell arithmetic (embedded-Q1 mode 1)
   int scaled = 0;     // assembled with learned per-sub-cell derivative scales
   bool csr_stale = false;  // block-space matrix: CSR values and FP32 stencil not yet derived from the stencil form
+  bool mg_stale = false;   // refilled since the cached hierarchy was computed (refreshed in place on its next use)
   dohp::DevBuf<double> q1tab;  // the assembly kernel's 1-D tables
   ~_p_dohpMat() { dohpMGDestroy(&mg); }
 };

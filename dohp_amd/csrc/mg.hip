@@ -636,16 +636,15 @@ struct _p_dohpMG {
   int coeff16 = 1;  // the FP32 V-cycle reads the FP16 coefficients
   DevBuf<double> cinv64;
   DevBuf<float> cinv32;
+  DevBuf<double> part;  // the power iteration's 256 partial sums and last norm
 };
 
 namespace {
 
-dErr estimate_lmax(MGLevel &lv, hipStream_t st, int its)
+// lmax of D^-1 A on a level by power iteration, in the level's work vectors (free outside a V-cycle).
+dErr estimate_lmax(MGLevel &lv, DevBuf<double> &part, hipStream_t st, int its)
 {
-  DevBuf<double> v, w, part;
-  DOHP_CALL(v.alloc(lv.n));
-  DOHP_CALL(w.alloc(lv.n));
-  DOHP_CALL(part.alloc(257));  // 256 partial sums and the last norm
+  DevBuf<double> &v = lv.r64, &w = lv.d64;
   hipLaunchKernelGGL(power_start_kernel, grid_of(lv.n), dim3(256), 0, st, lv.n, v.ptr);
   // the iterations stay on the device: one read of the last norm at the end
   hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, st, lv.n, v.ptr, part.ptr);
@@ -828,6 +827,43 @@ void vcycle(_p_dohpMG *mg, size_t l, const T *b, T *x, hipStream_t st)
 
 }  // namespace
 
+namespace {
+
+// The operators of every level from the finest level's stencil (L[0].A64): Galerkin products, half stencils in FP64,
+// FP32 and FP16, Jacobi scalings, eigenvalue estimates and the coarsest level's inverse, into the hierarchy's buffers.
+dErr mg_fill(_p_dohpMG *mg, hipStream_t st)
+{
+  for (size_t l = 1; l < mg->L.size(); l++) {
+    const MGLevel &f = mg->L[l - 1];
+    MGLevel &c = mg->L[l];
+    hipLaunchKernelGGL(rap_kernel, grid_of(c.n), dim3(256), 0, st, f.box, c.box, f.A64, c.own64.ptr);
+  }
+  for (size_t l = 0; l < mg->L.size(); l++) {
+    MGLevel &lv = mg->L[l];
+    const long n = lv.n;
+    hipLaunchKernelGGL(half_from_full_kernel, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.H64.ptr, lv.H32.ptr);
+    if (l + 1 < mg->L.size())
+      hipLaunchKernelGGL(half16_setup_kernel, box_grid(lv.box), box_block(), 0, st, lv.box, lv.A64, lv.H16.ptr,
+                         lv.dg16.ptr, lv.sc16.ptr);
+    hipLaunchKernelGGL(diag_inv_kernel<double>, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.dinv64.ptr);
+    hipLaunchKernelGGL(to_f32<float>, grid_of(n), dim3(256), 0, st, n, lv.dinv64.ptr, lv.dinv32.ptr);
+    if (l + 1 < mg->L.size()) DOHP_CALL(estimate_lmax(lv, mg->part, st, 15));
+  }
+  std::vector<double> inv;
+  DOHP_CALL(coarse_inverse(mg->L.back(), st, inv));
+  mg->ncoarse = (int)mg->L.back().n;
+  DOHP_CALL(mg->cinv64.upload(inv.data(), inv.size()));
+  std::vector<float> inv32(inv.begin(), inv.end());
+  DOHP_CALL(mg->cinv32.upload(inv32.data(), inv32.size()));
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    set_error("multigrid setup failed");
+    return dohpERR_HIP;
+  }
+  return 0;
+}
+
+}  // namespace
+
 extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohpMG *out, void *stream)
 {
   DOHP_CHECK(q1 && out, dohpERR_ARG, "null argument");
@@ -837,7 +873,7 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
   _p_dohpMG *mg = new _p_dohpMG;
   mg->degree = smooth_degree > 0 ? smooth_degree : 2;
   const long nmax = max_coarse > 0 ? max_coarse : 512;
-  dErr err = 0;
+  dErr err = mg->part.alloc(257);
   {
     MGLevel lv;
     lv.box = Box{q1->box[0], q1->box[1], q1->box[2]};
@@ -855,7 +891,6 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
     c.n = cb.n();
     err = c.own64.alloc(27 * c.n);
     if (err) break;
-    hipLaunchKernelGGL(rap_kernel, grid_of(c.n), dim3(256), 0, st, f.box, cb, f.A64, c.own64.ptr);
     c.A64 = c.own64.ptr;
     mg->L.push_back(std::move(c));
   }
@@ -878,33 +913,25 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
     if (!err && l + 1 < mg->L.size()) err = lv.dg16.alloc(n);
     if (!err && l + 1 < mg->L.size()) err = lv.sc16.alloc(n);
     if (!err && l + 1 < mg->L.size()) err = lv.t32.alloc(n);
-    if (err) break;
-    hipLaunchKernelGGL(half_from_full_kernel, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.H64.ptr, lv.H32.ptr);
-    if (l + 1 < mg->L.size())
-      hipLaunchKernelGGL(half16_setup_kernel, box_grid(lv.box), box_block(), 0, st, lv.box, lv.A64, lv.H16.ptr,
-                         lv.dg16.ptr, lv.sc16.ptr);
-    hipLaunchKernelGGL(diag_inv_kernel<double>, grid_of(n), dim3(256), 0, st, n, lv.A64, lv.dinv64.ptr);
-    hipLaunchKernelGGL(to_f32<float>, grid_of(n), dim3(256), 0, st, n, lv.dinv64.ptr, lv.dinv32.ptr);
-    if (l + 1 < mg->L.size()) err = estimate_lmax(lv, st, 15);
   }
-  if (!err) {
-    std::vector<double> inv;
-    err = coarse_inverse(mg->L.back(), st, inv);
-    mg->ncoarse = (int)mg->L.back().n;
-    if (!err) err = mg->cinv64.upload(inv.data(), inv.size());
-    std::vector<float> inv32(inv.begin(), inv.end());
-    if (!err) err = mg->cinv32.upload(inv32.data(), inv32.size());
-  }
-  if (!err && hipStreamSynchronize(st) != hipSuccess) {
-    set_error("multigrid setup failed");
-    err = dohpERR_HIP;
-  }
+  if (!err) err = mg_fill(mg, st);
   if (err) {
     delete mg;
     return err;
   }
   *out = mg;
   return 0;
+}
+
+extern "C" dErr dohpMGRefresh(dohpMG mg, dohpMat q1, void *stream)
+{
+  DOHP_CHECK(mg && q1, dohpERR_ARG, "null argument");
+  DOHP_CHECK(q1->st64.ptr, dohpERR_ARG, "multigrid needs the stencil form of a block-space Q1 matrix");
+  const Box &b = mg->L[0].box;
+  DOHP_CHECK(q1->box[0] == b.nx && q1->box[1] == b.ny && q1->box[2] == b.nz, dohpERR_ARG,
+             "multigrid refresh needs a matrix on the same box");
+  mg->L[0].A64 = q1->st64.ptr;
+  return mg_fill(mg, (hipStream_t)stream);
 }
 
 extern "C" dErr dohpMGGetInfo(dohpMG mg, int *nlevels, long *ncoarse, double *lmax_fine)

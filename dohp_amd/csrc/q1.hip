@@ -620,7 +620,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   hipStream_t st = (hipStream_t)stream;
   if (u) DOHP_CALL(dohpEllipFunctionState_Private(op, u, st));
   _p_dohpMat *A = *out;
-  if (A) dohpMGDestroy(&A->mg);  // new values: the cached hierarchy is stale
+  if (A) A->mg_stale = true;  // new values: the cached hierarchy is recomputed, in its own buffers, when next used
   if (!A) {
     A = new _p_dohpMat;
     dErr err = build_pattern(fs, A);

@@ -346,11 +346,14 @@ namespace {
 dErr hierarchy(dohpMat q1, int deg, hipStream_t st, dohpMG *mg)
 {
   deg = deg > 0 ? deg : (q1->corrected || q1->scaled ? 1 : 2);
-  if (!q1->mg || q1->mg_degree != deg) {
+  if (q1->mg && q1->mg_degree == deg) {
+    if (q1->mg_stale) DOHP_CALL(dohpMGRefresh(q1->mg, q1, st));
+  } else {
     dohpMGDestroy(&q1->mg);
     DOHP_CALL(dohpMGCreate(q1, deg, 0, &q1->mg, st));
     q1->mg_degree = deg;
   }
+  q1->mg_stale = false;
   *mg = q1->mg;
   return 0;
 }

@@ -269,7 +269,11 @@ class ScaledProblem:
         """Mean relative energy error of the learned V-cycle (Chebyshev smoothing of the given degree) over the pairs
         (shared optimal scale) and its gradient in the sub-cell scales added into gsig."""
         torch, C = self.torch, self.C
-        mg = C.MG(self.q1s, smooth_degree=smooth_degree)
+        if getattr(self, "_mg", None) is None or self._mg_degree != smooth_degree:
+            self._mg, self._mg_degree = C.MG(self.q1s, smooth_degree=smooth_degree), smooth_degree
+        else:  # the learned matrix was refilled (same box): recompute the hierarchy in place
+            self._mg.refresh(self.q1s)
+        mg = self._mg
         zs, terms = [], []
         for (r, e) in self.pairs:
             z = torch.zeros_like(r)

@@ -197,14 +197,17 @@ def newton(op, x, pc=None, rtol=1e-10, atol=1e-50, stol=1e-8, max_it=50, linesea
     ttol = rtol * fnorm
     if fnorm < atol:
         return x, hist, ksp_hist, 2
-    q1 = None
+    q1, mg = None, None
     for it in range(1, max_it + 1):
         if pc is None:
             M = None
         elif pc == "mg":
             from dohp_amd.capi import MG
             q1 = op.assemble_q1(x, mat=q1)
-            mg = MG(q1, smooth_degree=mg_degree)
+            if mg is None:
+                mg = MG(q1, smooth_degree=mg_degree)
+            else:  # the same box: the hierarchy is recomputed in its own buffers
+                mg.refresh(q1)
             if mg_fp32:
                 r32, z32 = torch.empty(x.shape, dtype=torch.float32, device=x.device), \
                     torch.empty(x.shape, dtype=torch.float32, device=x.device)

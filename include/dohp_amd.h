@@ -266,6 +266,9 @@ typedef struct _p_dohpMG *dohpMG;
  * most max_coarse nodes (0 = 512).  The V-cycle is symmetric (usable inside CG).  One apply at a
  * time per object: the levels' work vectors are shared. */
 dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohpMG *mg, void *stream);
+/* Recompute every level of mg from q1 refilled with new values on the same box (a Newton step's re-assembly), in
+ * the hierarchy's own buffers: no allocation (no counterpart in the reference, which rebuilds its PC). */
+dErr dohpMGRefresh(dohpMG mg, dohpMat q1, void *stream);
 dErr dohpMGGetInfo(dohpMG mg, int *nlevels, long *ncoarse, double *lmax_fine);
 /* Box dimensions and 27-point stencil [27][n] (FP64) of level `level` (0 = the Q1 matrix). */
 dErr dohpMGGetLevel(dohpMG mg, int level, int *box, double *stencil_host);

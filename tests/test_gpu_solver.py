@@ -277,3 +277,34 @@ def test_assembly_and_solves_are_reproducible(C):
             out.append((st["inner_its"], st["outer_its"], x.clone()))
         print("mixed", mixed, "its", out[0][:2], out[1][:2])
         assert out[0][:2] == out[1][:2] and torch.equal(out[0][2], out[1][2])
+
+
+def test_multigrid_refresh_matches_a_new_hierarchy(C):
+    """A hierarchy refreshed in place after its matrix was refilled at another state (a Newton step's re-assembly)
+    equals one built from scratch on the refilled matrix: every level's stencil, the eigenvalue estimate and the
+    V-cycle's output, bitwise."""
+    print("multigrid refresh", flush=True)
+    M, deg = (4, 5, 3), 3
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=3)
+    fs = C.FS(M, deg, V)
+    op = C.Ellip(fs, p=1.5, eps=0.3)
+    u0 = torch.as_tensor(splitmix64_uniform(1, fs.nowned) - 0.5, dtype=torch.float64, device="cuda")
+    u1 = torch.as_tensor(2 * splitmix64_uniform(2, fs.nowned) - 1, dtype=torch.float64, device="cuda")
+    q1 = op.assemble_q1(u0)
+    mg = C.MG(q1, smooth_degree=2, max_coarse=64)
+    lmax0 = mg.lmax
+    q1 = op.assemble_q1(u1, mat=q1)
+    mg.refresh(q1)
+    fresh = C.MG(q1, smooth_degree=2, max_coarse=64)
+    assert mg.nlevels == fresh.nlevels >= 2 and mg.lmax == fresh.lmax and mg.lmax != lmax0
+    for l in range(mg.nlevels):
+        np.testing.assert_array_equal(mg.level(l)[1], fresh.level(l)[1])
+    r = torch.as_tensor(splitmix64_uniform(3, fs.nowned) - 0.5, dtype=torch.float64, device="cuda")
+    z1, z2 = torch.zeros_like(r), torch.zeros_like(r)
+    mg.apply(r, z1)
+    fresh.apply(r, z2)
+    assert torch.equal(z1, z2)
+    z1, z2 = torch.zeros_like(r, dtype=torch.float32), torch.zeros_like(r, dtype=torch.float32)
+    mg.apply_f32(r.float(), z1)
+    fresh.apply_f32(r.float(), z2)
+    assert torch.equal(z1, z2)

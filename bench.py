@@ -649,6 +649,13 @@ def main():
         print(f"# {name} {t * 1e6:.1f} us", file=sys.stderr, flush=True)
         return t
 
+    # the same J v steps with the device NaN check's wait switched off (dohpEllipSetChecks 0, how the library's own
+    # Krylov loops apply the operator): the applications queue back to back
+    op.set_checks(0)
+    t_async = time_fn(torch, lambda: op.mat_mult(v, y, stream), args.steps)
+    op.set_checks(1)
+    variants["jv_f64_unchecked"] = {"us": round(t_async * 1e6, 1), "gdofs": round(fs.nowned / t_async / 1e9, 2)}
+    print(f"# Jv fp64 unchecked {t_async * 1e6:.1f} us", file=sys.stderr, flush=True)
     t_res = timed("residual_f64", lambda: op.function(u, f, stream), 8, True, 0, "res")
     # forming the quadrature data: the first application after a residual pays it
     t_pair = time_fn(torch, lambda: (op.function(u, f, stream), op.mat_mult(v, y, stream)), reps)

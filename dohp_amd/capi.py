@@ -109,6 +109,10 @@ class MG:
         check(_lib.dohpMGGetInfo(self.h, ctypes.byref(nl), ctypes.byref(nc), ctypes.byref(lm)))
         self.lmax = lm.value
 
+    def set_graphs(self, on):
+        """Replay V-cycles applied to the same vectors as HIP graphs (True, the default) or launch eagerly."""
+        check(_lib.dohpMGSetGraphs(self.h, int(bool(on))))
+
     def apply(self, r, z, stream=None):
         check(_lib.dohpMGApply(self.h, _ptr(r), _ptr(z), _stream(stream)))
 
@@ -150,6 +154,7 @@ _proto("dohpMGGetInfo", _vp, _vp, _vp, _vp)
 _proto("dohpMGGetLevel", _vp, ctypes.c_int, _vp, _vp)
 _proto("dohpMGApply", _vp, _vp, _vp, _vp)
 _proto("dohpMGRefresh", _vp, _vp, _vp)
+_proto("dohpMGSetGraphs", _vp, ctypes.c_int)
 _proto("dohpMGApplyF32", _vp, _vp, _vp, _vp)
 _proto("dohpMGResidual", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)
 _proto("dohpMGResidualF32", _vp, ctypes.c_int, _vp, _vp, _vp, _vp)

@@ -637,6 +637,30 @@ struct _p_dohpMG {
   DevBuf<double> cinv64;
   DevBuf<float> cinv32;
   DevBuf<double> part;  // the power iteration's 256 partial sums and last norm
+  // V-cycles replayed as HIP graphs: the coarse levels' kernels run for a few microseconds each, less than their
+  // launch costs the host, so the eager V-cycle leaves the GPU waiting; a graph is captured (on a private stream)
+  // per precision for the vectors it was applied to and replayed while they stay the same.
+  struct Graph {
+    const void *r = nullptr;  // the vectors of the captured graph, or of the last eager application
+    void *z = nullptr;
+    int coeff16 = -1;
+    hipGraphExec_t exec = nullptr;
+  };
+  Graph g64, g32;
+  int graphs = 1;
+  hipStream_t cap = nullptr;
+  void drop_graphs()
+  {
+    for (Graph *g : {&g64, &g32}) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      *g = Graph{};
+    }
+  }
+  ~_p_dohpMG()
+  {
+    drop_graphs();
+    if (cap) (void)hipStreamDestroy(cap);
+  }
 };
 
 namespace {
@@ -833,6 +857,7 @@ namespace {
 // FP32 and FP16, Jacobi scalings, eigenvalue estimates and the coarsest level's inverse, into the hierarchy's buffers.
 dErr mg_fill(_p_dohpMG *mg, hipStream_t st)
 {
+  mg->drop_graphs();  // their kernels carry this hierarchy's smoothing coefficients
   for (size_t l = 1; l < mg->L.size(); l++) {
     const MGLevel &f = mg->L[l - 1];
     MGLevel &c = mg->L[l];
@@ -959,23 +984,75 @@ extern "C" dErr dohpMGGetLevel(dohpMG mg, int level, int *box, double *stencil_h
   return 0;
 }
 
+namespace {
+
+// Applies the V-cycle `run(stream)` for vectors (r, z): replays the graph captured for them; the second consecutive
+// application to the same vectors captures it (a Krylov loop applies its preconditioner to the same work vectors
+// every iteration; one-off applications stay eager).  A capture that fails leaves the hierarchy on eager launches.
+template <class Run>
+dErr vcycle_graph(_p_dohpMG *mg, _p_dohpMG::Graph &g, const void *r, void *z, hipStream_t st, Run run)
+{
+  const bool same = g.r == r && g.z == z && g.coeff16 == mg->coeff16;
+  if (!mg->graphs || !same) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = _p_dohpMG::Graph{};
+    g.r = r;
+    g.z = z;
+    g.coeff16 = mg->coeff16;
+    run(st);
+    DOHP_HIP(hipGetLastError());
+    return 0;
+  }
+  if (!g.exec) {
+    if (!mg->cap) DOHP_HIP(hipStreamCreateWithFlags(&mg->cap, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    bool ok = hipStreamBeginCapture(mg->cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      run(mg->cap);
+      ok = hipStreamEndCapture(mg->cap, &graph) == hipSuccess && graph;
+    }
+    if (ok) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {  // eager from now on
+      (void)hipGetLastError();
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = _p_dohpMG::Graph{};
+      mg->graphs = 0;
+      run(st);
+      DOHP_HIP(hipGetLastError());
+      return 0;
+    }
+  }
+  DOHP_HIP(hipGraphLaunch(g.exec, st));
+  return 0;
+}
+
+}  // namespace
+
 extern "C" dErr dohpMGApply(dohpMG mg, const double *r, double *z, void *stream)
 {
   DOHP_CHECK(mg && r && z, dohpERR_ARG, "null argument");
-  vcycle<double>(mg, 0, r, z, (hipStream_t)stream);
-  DOHP_HIP(hipGetLastError());
+  return vcycle_graph(mg, mg->g64, r, z, (hipStream_t)stream,
+                      [&](hipStream_t s) { vcycle<double>(mg, 0, r, z, s); });
+}
+
+extern "C" dErr dohpMGSetGraphs(dohpMG mg, int on)
+{
+  DOHP_CHECK(mg && (on == 0 || on == 1), dohpERR_ARG, "graphs must be 0 (eager launches) or 1");
+  mg->drop_graphs();
+  mg->graphs = on;
   return 0;
 }
 
 extern "C" dErr dohpMGApplyF32(dohpMG mg, const float *r, float *z, void *stream)
 {
   DOHP_CHECK(mg && r && z, dohpERR_ARG, "null argument");
-  if (mg->coeff16 && mg->L.size() > 1)
-    vcycle16(mg, 0, r, z, (hipStream_t)stream);
-  else
-    vcycle<float>(mg, 0, r, z, (hipStream_t)stream);
-  DOHP_HIP(hipGetLastError());
-  return 0;
+  return vcycle_graph(mg, mg->g32, r, z, (hipStream_t)stream, [&](hipStream_t s) {
+    if (mg->coeff16 && mg->L.size() > 1)
+      vcycle16(mg, 0, r, z, s);
+    else
+      vcycle<float>(mg, 0, r, z, s);
+  });
 }
 
 extern "C" dErr dohpMGResidual(dohpMG mg, int level, const double *b, const double *x, double *r, void *stream)

@@ -269,9 +269,14 @@ dErr estimate_lmax(Ctx &c, dohpMat q1, const double *dinv, int its, double *lmax
 // PCG on A x = b (x holds the initial guess).  Stops at ||r|| <= rtol ||b|| or max_it.  When the
 // budget switches the learned correction off, CG restarts with the plain preconditioner from the
 // current iterate.  snap (FP64 only): x after every `every`-th iteration, at most nsnap copies.
+// flexible: beta in the Polak-Ribiere form z_{k+1}.(r_{k+1} - r_k) / z_k.r_k = -alpha z_{k+1}.q_k / z_k.r_k (one more
+// dot product), equal to the usual z_{k+1}.r_{k+1} / z_k.r_k when the preconditioner is symmetric and exact, and
+// robust when it is not: an FP32 V-cycle inside FP64 CG is symmetric only to FP32 rounding, which on the reference
+// arithmetic's matrix at 48^3, p = 6 sometimes doubled the iteration count (91 -> 193).
 template <typename T>
 dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double rtol, int max_it, int *its,
-         double *rnorm, double *bnorm, T *snap = nullptr, int nsnap = 0, int every = 0, int *nsnapped = nullptr)
+         double *rnorm, double *bnorm, T *snap = nullptr, int nsnap = 0, int every = 0, int *nsnapped = nullptr,
+         bool flexible = false)
 {
   PoolBuf<T> r, z, p, q;
   DOHP_CALL(r.alloc(*c.pool, c.n));
@@ -288,7 +293,7 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double
   DOHP_CHECK(std::isfinite(bb) && std::isfinite(rr), dohpERR_NAN, "PCG: non-finite right-hand side or initial residual");
   const double tol = rtol * std::sqrt(bb);
   int k = 0, ns = 0;
-  double rz = 0;
+  double rz = 0, alpha = 0;
   bool restart = false;
   auto precondition = [&]() -> dErr {
     bool &learned_on = M.low ? M.low->learned_on : M.learned_on;
@@ -309,7 +314,7 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double
     DOHP_CALL(Ops<T>::A(op, p.ptr, q.ptr, c.st));
     double pq;
     DOHP_CALL(dot(c, p.ptr, q.ptr, &pq));
-    const double alpha = rz / pq;
+    alpha = rz / pq;
     DOHP_CALL(axpby<T>(c, (T)alpha, p.ptr, T(1), x));
     DOHP_CALL(axpby<T>(c, (T)-alpha, q.ptr, T(1), r.ptr));
     DOHP_CALL(dot(c, r.ptr, r.ptr, &rr));
@@ -328,7 +333,13 @@ dErr pcg(Ctx &c, dohpEllip op, Prec<T> &M, Budget &bud, const T *b, T *x, double
       DOHP_HIP(hipMemcpyAsync(p.ptr, z.ptr, sizeof(T) * c.n, hipMemcpyDeviceToDevice, c.st));
       restart = false;
     } else {
-      DOHP_CALL(axpby<T>(c, T(1), z.ptr, (T)(rz_new / rz), p.ptr));
+      double beta = rz_new / rz;
+      if (flexible) {
+        double zq;
+        DOHP_CALL(dot(c, z.ptr, q.ptr, &zq));
+        beta = -alpha * zq / rz;
+      }
+      DOHP_CALL(axpby<T>(c, T(1), z.ptr, (T)beta, p.ptr));
     }
     rz = rz_new;
   }
@@ -451,7 +462,8 @@ dErr solve_impl(dohpEllip op, dohpMat q1, dohpMLP mlp, const double *b, double *
     M.lz = lz.ptr;
     int its;
     double rn, bn;
-    DOHP_CALL(pcg<double>(c, op, M, bud, b, x, opts->rtol, opts->max_it, &its, &rn, &bn, snap, nsnap, every, nsnapped));
+    DOHP_CALL(pcg<double>(c, op, M, bud, b, x, opts->rtol, opts->max_it, &its, &rn, &bn, snap, nsnap, every, nsnapped,
+                          true));
     stats->inner_its = its;
     stats->outer_its = 1;
     stats->rnorm = rn;

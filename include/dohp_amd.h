@@ -269,6 +269,10 @@ dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohpMG *mg, voi
 /* Recompute every level of mg from q1 refilled with new values on the same box (a Newton step's re-assembly), in
  * the hierarchy's own buffers: no allocation (no counterpart in the reference, which rebuilds its PC). */
 dErr dohpMGRefresh(dohpMG mg, dohpMat q1, void *stream);
+/* V-cycles as HIP graphs (no counterpart in the reference): with on = 1 (the default) the second consecutive
+ * application to the same vectors captures the V-cycle and later ones replay it (the coarse levels' kernels are
+ * shorter than their launches); 0 launches every kernel eagerly. */
+dErr dohpMGSetGraphs(dohpMG mg, int on);
 dErr dohpMGGetInfo(dohpMG mg, int *nlevels, long *ncoarse, double *lmax_fine);
 /* Box dimensions and 27-point stencil [27][n] (FP64) of level `level` (0 = the Q1 matrix). */
 dErr dohpMGGetLevel(dohpMG mg, int level, int *box, double *stencil_host);

@@ -308,3 +308,39 @@ def test_multigrid_refresh_matches_a_new_hierarchy(C):
     mg.apply_f32(r.float(), z1)
     fresh.apply_f32(r.float(), z2)
     assert torch.equal(z1, z2)
+
+
+def test_vcycle_graph_replays_match_eager_launches(C):
+    """The V-cycle replayed as a HIP graph (captured on the second application to the same vectors) gives, on every
+    replay, the bits of the eager V-cycle, in FP64 and FP32 (FP16 and FP32 coefficients), with the program's own work
+    and synchronizations between the replays; a refresh of the hierarchy drops the graphs (new coefficients)."""
+    print("vcycle graphs", flush=True)
+    fs, op, q1, xs, b = _problem(C, (6, 7, 5), 5)
+    mg = C.MG(q1, smooth_degree=2, max_coarse=64)
+    eager = C.MG(q1, smooth_degree=2, max_coarse=64)
+    eager.set_graphs(False)
+    n = fs.nowned
+    for dtype, apply_name, coeffs in ((torch.float64, "apply", True), (torch.float32, "apply_f32", True),
+                                      (torch.float32, "apply_f32", False)):
+        mg.set_coefficients(coeffs)
+        eager.set_coefficients(coeffs)
+        r = torch.zeros(n, dtype=dtype, device="cuda")
+        z = torch.zeros_like(r)
+        zref = torch.zeros_like(r)
+        for rep in range(6):
+            r.copy_(torch.as_tensor(splitmix64_uniform(100 + rep, n) - 0.5, dtype=dtype, device="cuda"))
+            getattr(mg, apply_name)(r, z)
+            getattr(eager, apply_name)(r, zref)
+            torch.cuda.synchronize()
+            assert torch.equal(z, zref), (dtype, coeffs, rep)
+            assert float(torch.linalg.norm(z)) > 0
+    q1 = op.assemble_q1(xs, mat=q1)
+    mg.refresh(q1)
+    eager.refresh(q1)
+    r = torch.as_tensor(splitmix64_uniform(7, n) - 0.5, dtype=torch.float64, device="cuda")
+    z, zref = torch.zeros_like(r), torch.zeros_like(r)
+    for rep in range(3):
+        mg.apply(r, z)
+        eager.apply(r, zref)
+        torch.cuda.synchronize()
+        assert torch.equal(z, zref)

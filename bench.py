@@ -94,8 +94,13 @@ def alg_flops(n, kind):
     push-forward and its transpose 18 per point each; the pointwise step 8 (linear), 20 (Jacobian) or 40 (residual)
     per point.
         F_lin = 36 n^4 + (18 + 8 + 18) n^3 + G,   F_Jv = 36 n^4 + (18 + 20 + 18) n^3 + G,
-        F_res = 48 n^4 + (18 + 40 + 18) n^3 + G,  G = 4 C(2 -> n, 3) + 44 n^3."""
+        F_res = 48 n^4 + (18 + 40 + 18) n^3 + G,  G = 4 C(2 -> n, 3) + 44 n^3.
+    kind "jv_ref": the Jacobian action as the reference's iterator performs it, value and gradient of the direction
+    and their transposes (eight n^3 -> n^3 contractions, 48 n^4) with the same geometry, push-forwards and pointwise step
+    (172 kflop per element at p = 6; F_Jv, 143 kflop, keeps only what the action needs with lambda = 0)."""
     g = 4 * _contraction_flops(2, n, 3) + 44 * n ** 3
+    if kind == "jv_ref":
+        return 48 * n ** 4 + 56 * n ** 3 + g
     if kind == "lin":
         return 36 * n ** 4 + 44 * n ** 3 + g
     if kind == "jv":
@@ -713,8 +718,11 @@ def main():
     if traffic:
         roofline["traffic_over_algorithmic"] = round(traffic / B, 3)
     F = alg_flops(n, "jv") * fs.nelem
+    F_ref = alg_flops(n, "jv_ref") * fs.nelem
     compute = {"bound": "fp64", "achieved": round(F / t_k / 1e12, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                "frac": round(F / t_k / 1e12 / FP64_PEAK_TFLOPS, 4), "algorithmic_flops": int(F),
+               "frac_reference_count": round(F_ref / t_k / 1e12 / FP64_PEAK_TFLOPS, 4),
+               "reference_count_flops": int(F_ref),
                "flop_model": "F_Jv = 36 n^4 + 56 n^3 + 4 C(2->n, 3) + 44 n^3 per element, C(P->Q, D) = "
                              "2 D (Q P^3 + Q^2 P^2 + Q^3 P): the reference's gradient, push-forward, pointwise "
                              "Jacobian, transposes and recomputed geometry"}

@@ -76,3 +76,5 @@ def test_headline_flop_model():
     # J v at p = 6: six gradient passes, push-forwards, pointwise, geometry = 143,248 flops per element
     assert bench.alg_flops(7, "jv") == 6 * 14406 + 56 * 343 + 4 * bench._contraction_flops(2, 7, 3) + 44 * 343
     assert bench.alg_flops(7, "jv") == 143248
+    # the reference's iterator also interpolates the direction's value and its transpose: two more contractions
+    assert bench.alg_flops(7, "jv_ref") == bench.alg_flops(7, "jv") + 2 * 14406 == 172060

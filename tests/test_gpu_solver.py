@@ -308,6 +308,13 @@ def test_multigrid_refresh_matches_a_new_hierarchy(C):
     mg.apply_f32(r.float(), z1)
     fresh.apply_f32(r.float(), z2)
     assert torch.equal(z1, z2)
+    # a matrix on another box is refused
+    fs2 = C.FS((4, 4, 3), deg, block_vertices((4, 4, 3), box=(0, 1, 0, 1, 0, 1)))
+    op2 = C.Ellip(fs2, p=2.0)
+    q2 = op2.assemble_q1(torch.zeros(fs2.nowned, dtype=torch.float64, device="cuda"))
+    with pytest.raises(C.DohpError) as ei:
+        mg.refresh(q2)
+    assert ei.value.code == C.ERR_ARG and "same box" in str(ei.value)
 
 
 def test_vcycle_graph_replays_match_eager_launches(C):

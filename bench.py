@@ -700,6 +700,31 @@ def main():
     timed("jv_f32_recomputed", lambda: op.mat_mult_f32(v32, y32, stream), 4, True, 0)
     tuning.update({f"jacobian_{'f32' if f else 'f64'}": op.tuning("jacobian", f) for f in (0, 1)})
     variants["tuning"] = tuning
+    # the matrix-core shape (the one-dimensional contractions as 16x16x4 MFMA products in the kernel's
+    # precision), pinned, against the shape the tuner kept: time and relative difference of the results
+    mfma = {}
+    y_ref = torch.zeros_like(y)
+    op.set_storage(1)
+    lin.set_storage(0)
+    for label, o, mode, fp32, apply in (
+            ("jv_f64_stored", op, "stored", 0, lambda o, out: o.mat_mult(v, out, stream)),
+            ("jv_f32_stored", op, "stored", 1, lambda o, out: o.mat_mult_f32(v32, out, stream)),
+            ("linear_f32", lin, "linear", 1, lambda o, out: o.mat_mult_f32(v32, out, stream))):
+        out = y if fp32 == 0 else y32
+        ref = y_ref if fp32 == 0 else torch.zeros_like(y32)
+        apply(o, ref)
+        t_chosen = time_fn(torch, lambda: apply(o, ref), 4)
+        chosen = o.tuning(mode, fp32)[0]
+        o.set_shape(mode, fp32, 4)
+        apply(o, out)
+        t = time_fn(torch, lambda: apply(o, out), 4)
+        torch.cuda.synchronize()
+        o.set_shape(mode, fp32, chosen)
+        mfma[label] = {"us": round(t * 1e6, 1), "tuned_shape_us": round(t_chosen * 1e6, 1),
+                       "rel_diff": float(torch.linalg.norm((out - ref).double()) / torch.linalg.norm(ref.double()))}
+    variants["mfma_shape"] = mfma
+    lin.set_storage(2)
+    del y_ref
     op.set_storage(2)
 
     B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True)

@@ -488,9 +488,10 @@ class Ellip:
 
     def tuning(self, mode, fp32=False):
         """(chosen shape, ms of [four-wavefront, one-wavefront, persistent one-wavefront,
-        persistent four-wavefront]) of a mode: "linear" and "jacobian" recompute the geometry (storage 0),
-        "stored" applies either operator from its quadrature data (storage 1)."""
-        sh, ms = ctypes.c_int(), (ctypes.c_float * 4)()
+        persistent four-wavefront, four-wavefront on the matrix cores (degrees 3..7)]) of a mode: "linear"
+        and "jacobian" recompute the geometry (storage 0), "stored" applies either operator from its
+        quadrature data (storage 1)."""
+        sh, ms = ctypes.c_int(), (ctypes.c_float * 5)()
         check(_lib.dohpEllipGetTuning(self.h, self._TUNE_MODES[mode], int(fp32), ctypes.byref(sh), ms))
         return sh.value, [round(v, 4) for v in ms]
 

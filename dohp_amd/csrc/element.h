@@ -17,6 +17,9 @@ namespace {
 // (or linear operator) that reads it (MODE_JQD).
 enum { MODE_LIN = 0, MODE_FUN = 1, MODE_JAC = 2, MODE_FRC = 3, MODE_QD = 4, MODE_JQD = 5 };
 constexpr int kModes = 6;
+// launch shapes of the fused kernels the tuner chooses from (four-wavefront, one-wavefront, their
+// persistent forms, and the four-wavefront shape with the contractions on the matrix cores)
+constexpr int kShapes = 5;
 
 
 struct Phys {
@@ -310,6 +313,129 @@ __device__ __forceinline__ void backward(const T *tab, T *A, T *GX, T *GY, int a
   tile_sync<WS>();
 }
 
+// ---- Line passes on the matrix cores (n = 5..8) ----
+//
+// A one-dimensional pass applies an n x n operator (or two stacked) to every line of the
+// workgroup's tiles along one direction: C[o][L] = sum_k Op[o][k] In[k][L].  Each wavefront takes
+// the 64 lines of its own threads as four blocks of 16 columns of one 16x16x4 MFMA product per
+// K step (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64, the precision of the kernel, so nothing is
+// rounded below it).  Operand layout of the 16x16x4 products: lane l holds A[l % 16][l / 16] of the
+// operator, B[l / 16][l % 16] of the lines and, after the product, C[row(l / 16, r)][l % 16],
+// r = 0..3, with row = 4 (l / 16) + r in FP32 and l / 16 + 4 r in FP64 (measured on gfx950).  Line inputs are one or two fields (K = 8 per field: positions s = 0..n-1 along the
+// line, the rest zero), outputs one or two fields (rows 0..7 into the first, 8..15 into the
+// second).  A pass reads and writes only its own lines, so it can update a field in place; passes
+// in different directions are separated by workgroup barriers.  Every lane of the wavefront must
+// be active (the workgroup has 256 threads, one per line; lines past the elements are a pad tile).
+constexpr int kMfmaMaxN = 8;
+enum { MF_B = 0, MF_BDB = 128, MF_BT = 256, MF_BTDBT = 384, kMfmaFrag = 640 };  // operator fragments [t][64]
+
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<float> {
+  typedef float acc __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc mma(float a, float b, acc c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static constexpr __device__ int row(int kq, int r) { return 4 * kq + r; }
+};
+template <>
+struct Mfma16<double> {
+  typedef double acc __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc mma(double a, double b, acc c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static constexpr __device__ int row(int kq, int r) { return kq + 4 * r; }
+};
+
+// One pass of this wavefront's lines in direction DIR; own = the first point of the thread's own line
+// in the tiles (the line of its lane's column in block j is lane 16 j + l % 16's own line), frag = the
+// operator's fragments.  (Issuing the four blocks' reads together, then four independent product
+// chains, measured slower: more registers, the same matrix-pipe time.)
+template <int N, int DIR, typename T, int NIN, int NOUT>
+__device__ __forceinline__ void mfma_lines(const T *__restrict__ frag, const T *in0, const T *in1, T *out0, T *out1,
+                                           int own, int lane)
+{
+  static_assert(N <= kMfmaMaxN, "the line pass holds at most 8 points per field");
+  constexpr int STR = DIR == DIR_Z ? 1 : DIR == DIR_Y ? N : N * N, KS = 2 * NIN;
+  typedef typename Mfma16<T>::acc acc;
+  const int kq = lane >> 4;
+  T op[KS];
+#pragma unroll
+  for (int t = 0; t < KS; t++) op[t] = frag[t * 64 + lane];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int base = __builtin_amdgcn_ds_bpermute((16 * j + (lane & 15)) << 2, own);
+    acc c = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int t = 0; t < KS; t++) {
+      const int s = 4 * (t & 1) + kq;
+      const T *src = t < 2 ? in0 : in1;
+      const T v = src[base + (s < N ? s : N - 1) * STR];  // (the read stays on the line; past it the value is 0)
+      c = Mfma16<T>::mma(op[t], s < N ? v : T(0), c);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int o = Mfma16<T>::row(kq, r), s = o & 7;
+      T *dst = NOUT == 2 && o >= 8 ? out1 : out0;
+      if ((NOUT == 2 || o < 8) && s < N) dst[base + s * STR] = c[r];
+    }
+  }
+}
+
+// forward() on the matrix cores: z (B), y (B and Dh B), x (B and Dh B; B of GY); the z derivative
+// of the thread's own line stays on the vector units.  TA, TGX, TGY are the workgroup's tiles, le the
+// thread's tile.  Called by all 256 threads.
+template <int N, typename T>
+__device__ __forceinline__ void forward_mfma(const T *tab, const T *frag, T *TA, T *TGX, T *TGY, int le, int a,
+                                             int b, T (&q)[N], T (&gx)[N], T (&gy)[N], T (&gz)[N])
+{
+  const int lane = threadIdx.x & 63, e0 = le * N * N * N;
+  const int bz = e0 + line_at<N, DIR_Z>(a, b, 0), by = e0 + line_at<N, DIR_Y>(a, b, 0),
+            bx = e0 + line_at<N, DIR_X>(a, b, 0);
+  mfma_lines<N, DIR_Z, T, 1, 1>(frag + MF_B, TA, TA, TA, TA, bz, lane);
+  __syncthreads();
+  mfma_lines<N, DIR_Y, T, 1, 2>(frag + MF_BDB, TA, TA, TA, TGY, by, lane);
+  __syncthreads();
+  mfma_lines<N, DIR_X, T, 1, 2>(frag + MF_BDB, TA, TA, TA, TGX, bx, lane);
+  mfma_lines<N, DIR_X, T, 1, 1>(frag + MF_B, TGY, TGY, TGY, TGY, bx, lane);
+  __syncthreads();
+  ld_line<N, DIR_Z>(&TA[e0], a, b, q);
+  eo_apply<N, -1>(tab, OP_D, q, gz);
+  ld_line<N, DIR_Z>(&TGX[e0], a, b, gx);
+  ld_line<N, DIR_Z>(&TGY[e0], a, b, gy);
+}
+
+// backward() on the matrix cores: the z transposed derivative of the thread's own line on the
+// vector units, then x (B^T of the sum plus (Dh B)^T of d0; B^T of d1), y (B^T plus (Dh B)^T of
+// d1), z (B^T).  Ends with a barrier; the result is in TA.
+template <int N, typename T>
+__device__ __forceinline__ void backward_mfma(const T *tab, const T *frag, T *TA, T *TGX, T *TGY, int le, int a,
+                                              int b, const T (&v)[N], const T (&d0)[N], const T (&d1)[N],
+                                              const T (&d2)[N])
+{
+  const int lane = threadIdx.x & 63, e0 = le * N * N * N;
+  const int bz = e0 + line_at<N, DIR_Z>(a, b, 0), by = e0 + line_at<N, DIR_Y>(a, b, 0),
+            bx = e0 + line_at<N, DIR_X>(a, b, 0);
+  T w[N];
+  eo_apply<N, -1>(tab, OP_DT, d2, w);
+#pragma unroll
+  for (int s = 0; s < N; s++) w[s] += v[s];
+  st_line<N, DIR_Z>(&TA[e0], a, b, w);
+  st_line<N, DIR_Z>(&TGY[e0], a, b, d1);
+  st_line<N, DIR_Z>(&TGX[e0], a, b, d0);
+  __syncthreads();
+  mfma_lines<N, DIR_X, T, 2, 1>(frag + MF_BTDBT, TA, TGX, TA, TA, bx, lane);
+  mfma_lines<N, DIR_X, T, 1, 1>(frag + MF_BT, TGY, TGY, TGY, TGY, bx, lane);
+  __syncthreads();
+  mfma_lines<N, DIR_Y, T, 2, 1>(frag + MF_BTDBT, TA, TGY, TA, TA, by, lane);
+  __syncthreads();
+  mfma_lines<N, DIR_Z, T, 1, 1>(frag + MF_BT, TA, TA, TA, TA, bz, lane);
+  __syncthreads();
+}
+
 // x^e for x > 0.  Quarter-integer exponents (ph.qpow = 4e in [-8, 8]) use the hardware
 // reciprocal square root refined by Newton steps; other exponents call pow.
 __device__ inline double rsqrt_nr(double x)
@@ -429,13 +555,44 @@ __global__ void min_det_kernel(int N, const int *__restrict__ everts, const doub
 
 // B, Dh, B^T, Dh^T, Dh B, (Dh B)^T of an n-point rule in the even-odd form of eo_apply,
 // kEoOps * eo_size(n) entries; B and Dh are [q][m] row-major.
-inline std::vector<double> eo_tables(int N, const std::vector<double> &B, const std::vector<double> &Dh)
+inline std::vector<double> db_matrix(int N, const std::vector<double> &B, const std::vector<double> &Dh)
 {
-  const int H = N / 2, S = eo_size(N);
   std::vector<double> DB(N * N, 0.0);
   for (int q = 0; q < N; q++)
     for (int m = 0; m < N; m++)
       for (int j = 0; j < N; j++) DB[q * N + m] += Dh[q * N + j] * B[j * N + m];
+  return DB;
+}
+
+// Operator fragments of the matrix-core line passes (kMfmaFrag entries, [set + t * 64 + lane]): lane l
+// holds Op[o = l % 16][k = 4 t + l / 16] of
+//   MF_B      B (rows o < n),                          K = one field
+//   MF_BDB    B in rows 0..7, Dh B in rows 8..15,      K = one field
+//   MF_BT     B^T,                                     K = one field
+//   MF_BTDBT  B^T over the first field, (Dh B)^T over the second (t = 2, 3)
+// with positions past n zero.
+inline std::vector<double> mfma_frags(int N, const std::vector<double> &B, const std::vector<double> &Dh)
+{
+  const std::vector<double> DB = db_matrix(N, B, Dh);
+  std::vector<double> f(kMfmaFrag, 0.0);
+  for (int t = 0; t < 4; t++)
+    for (int l = 0; l < 64; l++) {
+      const int o = l % 16, s = 4 * (t & 1) + l / 16;
+      if (s >= N) continue;
+      if (t < 2) {
+        if (o < N) f[MF_B + t * 64 + l] = B[o * N + s];
+        if ((o & 7) < N) f[MF_BDB + t * 64 + l] = (o < 8 ? B : DB)[(o & 7) * N + s];
+        if (o < N) f[MF_BT + t * 64 + l] = B[s * N + o];
+      }
+      if (o < N) f[MF_BTDBT + t * 64 + l] = (t < 2 ? B : DB)[s * N + o];
+    }
+  return f;
+}
+
+inline std::vector<double> eo_tables(int N, const std::vector<double> &B, const std::vector<double> &Dh)
+{
+  const int H = N / 2, S = eo_size(N);
+  const std::vector<double> DB = db_matrix(N, B, Dh);
   std::vector<double> t(kEoOps * S, 0.0);
   for (int opi = 0; opi < kEoOps; opi++) {
     auto A = [&](int q, int m) {

@@ -68,8 +68,10 @@ constexpr bool jac_state_first()
   return sizeof(T) == 4 || N >= 9;
 }
 
-template <int N, typename T, int MODE, int EPB>
-__global__ void __launch_bounds__(wg_threads(N, EPB))
+// MF: the contractions as line passes on the matrix cores (n <= 8; 256 threads, one per line of the
+// workgroup's tiles, lines past the elements in a pad tile).
+template <int N, typename T, int MODE, int EPB, bool MF = false>
+__global__ void __launch_bounds__(MF ? 256 : wg_threads(N, EPB))
 __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <= 7 ? 3 : 1))) ellip_kernel(const int *__restrict__ elist, long ne, BoxMap bm,
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
                                                     const T *__restrict__ ustate, const T *__restrict__ xdir,
@@ -77,15 +79,18 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
                                                     const T *__restrict__ gtab, const double *__restrict__ gwphi, Phys ph,
                                                     int *__restrict__ nanflag)
 {
-  constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
-  __shared__ T TA[EPB * N3], TGX[EPB * N3], TGY[EPB * N3];
+  static_assert(!MF || (N <= kMfmaMaxN && EPB * N * N <= 256 && MODE != MODE_QD && MODE != MODE_FRC),
+                "matrix-core passes: n <= 8, one line per thread, modes that scatter");
+  constexpr int NN = N * N, N3 = NN * N, NT = MF ? 256 : wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
+  constexpr int ET = EPB + (MF ? 1 : 0);  // element tiles (the pad tile last)
+  __shared__ T TA[ET * N3], TGX[ET * N3], TGY[ET * N3];
   __shared__ int IDX[EPB * N3];
-  __shared__ double VX[EPB * 8 * 4];
-  __shared__ int EB[EPB * 4];  // per element slot: first lattice point, 1 if the slot holds an element
+  __shared__ double VX[ET * 8 * 4];
+  __shared__ int EB[ET * 4];  // per element slot: first lattice point, 1 if the slot holds an element
 
   const long lb = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
-  for (int t = tid; t < EPB * 8; t += NT) {
+  for (int t = tid; t < ET * 8; t += NT) {
     const int l = t >> 3, vt = t & 7;
     const long es = lb * EPB + l;
     double *dst = &VX[(l * 8 + vt) * 4];
@@ -153,7 +158,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
       for (int s = 0; s < NS; s++) uv[s] = ustate[lat[s] > 0 ? lat[s] : 0];
     }
     if constexpr (MODE == MODE_JQD) {  // line and element as in the body; an empty slot reads element 0
-      const int ln = tid % (EPB * NN), l = ln / NN;
+      const int ln = MF ? tid : tid % (EPB * NN), l = ln / NN;
       const long es = lb * EPB + l;
       const long el = es < ne ? (elist ? (long)elist[es] : es) : 0;
       const T *q = ustate + el * (ph.lambda != 0 ? 7 : 6) * N3 + (ln - l * NN);
@@ -211,6 +216,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_LIN ? 3 : 1)))
   constexpr int NN = N * N, N3 = NN * N, NT = wg_threads(N, EPB), NS = (EPB * N3 + NT - 1) / NT;
   // the Jacobian action prefetches its two fields (state, direction) only in FP32 (in FP64 the
   // registers cost occupancy)
+  constexpr bool MF = false;  // (the matrix-core passes are a shape of the non-persistent kernel)
   constexpr bool PREFETCH_VALUES = MODE != MODE_JAC || sizeof(T) == 4;
   constexpr bool GATHER_X = MODE == MODE_LIN || MODE == MODE_JQD;  // else the state
   constexpr bool PREFETCH_STATE = MODE == MODE_JAC && PREFETCH_VALUES;
@@ -374,7 +380,7 @@ struct _p_dohpEllip {
   int q1mode = 0;   // embedded-Q1 assembly: 0 = the reference's arithmetic, 1 = corrected sub-cell derivatives
   bool has_state = false;
   int shape[kModes][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};  // tuned per [mode][FP32]
-  float shape_ms[kModes][2][4] = {};
+  float shape_ms[kModes][2][kShapes] = {};
   // Jacobian (or linear operator) quadrature data, 6 values per point (7 with the Bratu term), formed
   // on the first application after each residual (the linear operator's once).  storage: 0 = never,
   // 1 = always, 2 = the faster of the two, timed on the first application in each precision (use_qd:
@@ -449,21 +455,21 @@ BoxMap box_map(const _p_dohpFS *fs)
   return m;
 }
 
-template <int N, typename T, int MODE, int EPB>
+template <int N, typename T, int MODE, int EPB, bool MF = false>
 void launch_shape(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st, const Phys &ph)
 {
   const dohpFS fs = op->fs;
-  const dim3 block(wg_threads(N, EPB));
+  const dim3 block(MF ? 256 : wg_threads(N, EPB));
   if (op->scatter == 1) {
     const long ne = fs->nelem;
-    hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB>), dim3((ne + EPB - 1) / EPB), block, 0, st, nullptr, ne,
+    hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB, MF>), dim3((ne + EPB - 1) / EPB), block, 0, st, nullptr, ne,
                        box_map(fs), fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 1, gtab<T>(op),
                        op->wphi.ptr, ph, op->nanflag.ptr);
   } else {
     for (int col = 0; col < 8; col++) {
       const long ne = fs->color_off[col + 1] - fs->color_off[col];
       if (!ne) continue;
-      hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB>), dim3((ne + EPB - 1) / EPB), block, 0, st,
+      hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB, MF>), dim3((ne + EPB - 1) / EPB), block, 0, st,
                          fs->d_color_elems.ptr + fs->color_off[col], ne, box_map(fs), fs->d_vcoords.ptr,
                          op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), op->wphi.ptr, ph, op->nanflag.ptr);
     }
@@ -496,18 +502,24 @@ dErr launch_persistent(const _p_dohpEllip *op, const T *ust, const T *x, T *y, h
   return 0;
 }
 
+// Degrees whose fused kernels have the matrix-core shape (shape 4).
+constexpr bool has_mfma_shape(int N) { return N >= 4 && N <= kMfmaMaxN; }
+
 template <int N, typename T, int MODE>
 dErr launch_variant(const _p_dohpEllip *op, int v, const T *ust, const T *x, T *y, hipStream_t st, const Phys &ph)
 {
   if (v == 0) launch_shape<N, T, MODE, epb_wide(N)>(op, ust, x, y, st, ph);
   else if (v == 1) launch_shape<N, T, MODE, epb_narrow(N)>(op, ust, x, y, st, ph);
   else if (v == 2) DOHP_CALL((launch_persistent<N, T, MODE, epb_narrow(N)>(op, ust, x, y, st, ph)));
-  else DOHP_CALL((launch_persistent<N, T, MODE, epb_wide(N)>(op, ust, x, y, st, ph)));
+  else if (v == 3) DOHP_CALL((launch_persistent<N, T, MODE, epb_wide(N)>(op, ust, x, y, st, ph)));
+  else if constexpr (has_mfma_shape(N)) launch_shape<N, T, MODE, epb_wide(N), true>(op, ust, x, y, st, ph);
+  else DOHP_CHECK(false, dohpERR_ARG, "no matrix-core shape at this degree");
   return 0;
 }
 
 // Autotuner: the first application of each (mode, precision) times the workgroup shapes (four
-// wavefronts, one wavefront, and the persistent forms of both) on a scratch output and keeps the fastest
+// wavefronts, one wavefront, the persistent forms of both, and in FP32 from n = 4 to 8 the
+// four-wavefront shape with the contractions on the matrix cores) on a scratch output and keeps the fastest
 // (the choice is part of the operator; results do not depend on it beyond the summation order
 // of the atomic scatter).
 template <int N, typename T, int MODE>
@@ -515,7 +527,7 @@ dErr launch_n(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
 {
   const Phys ph = make_phys(op);
   int &shape = op->shape[MODE][sizeof(T) == 4];
-  if (op->scatter != 1 && shape >= 2) shape = -1;  // the persistent forms need the atomic scatter
+  if (op->scatter != 1 && (shape == 2 || shape == 3)) shape = -1;  // the persistent forms need the atomic scatter
   if (shape < 0) {
     const long n = op->fs->nowned;
     DevBuf<T> scratch;
@@ -524,8 +536,11 @@ dErr launch_n(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
     DOHP_HIP(hipEventCreate(&e0));
     DOHP_HIP(hipEventCreate(&e1));
     float best = 0;
-    const int nvar = op->scatter == 1 ? 4 : 2;
-    for (int v = 0; v < nvar; v++) {
+    for (int v = 0; v < kShapes; v++) {
+      if ((v == 2 || v == 3) && op->scatter != 1) continue;
+      // (the matrix-core shape is timed in FP32 only: FP64 MFMA issues at the FP64 vector rate, so its padded
+      // 16x16 tiles cannot win there; it stays available through dohpEllipSetShape)
+      if (v == 4 && (!has_mfma_shape(N) || sizeof(T) != 4)) continue;
       float ms = 0;
       for (int rep = 0; rep < 3; rep++) {  // the first run of each shape is a warm-up
         DOHP_HIP(hipEventRecord(e0, st));
@@ -700,8 +715,12 @@ extern "C" dErr dohpEllipCreate(dohpFS fs, const dohpEllipParam *prm, dohpEllip 
         err = dohpERR_NEGATIVE_JACOBIAN;
       }
   }
-  {  // B, Dh, B^T, Dh^T in even-odd form (see eo_apply)
-    const std::vector<double> t = eo_tables(N, op->B, op->Dh);
+  {  // B, Dh, B^T, Dh^T in even-odd form (see eo_apply), then the matrix-core line passes' fragments
+    std::vector<double> t = eo_tables(N, op->B, op->Dh);
+    if (N <= kMfmaMaxN) {
+      const std::vector<double> f = mfma_frags(N, op->B, op->Dh);
+      t.insert(t.end(), f.begin(), f.end());
+    }
     std::vector<float> tf(t.begin(), t.end());
     if (!err) err = op->tab64.upload(t.data(), t.size());
     if (!err) err = op->tab32.upload(tf.data(), tf.size());
@@ -753,15 +772,18 @@ extern "C" dErr dohpEllipGetTuning(dohpEllip op, int mode, int fp32, int *shape,
   const int m = kApiMode[mode];
   if (shape) *shape = op->shape[m][fp32];
   if (ms)
-    for (int v = 0; v < 4; v++) ms[v] = op->shape_ms[m][fp32][v];
+    for (int v = 0; v < kShapes; v++) ms[v] = op->shape_ms[m][fp32][v];
   return 0;
 }
 
 extern "C" dErr dohpEllipSetShape(dohpEllip op, int mode, int fp32, int shape)
 {
-  DOHP_CHECK(op && mode >= 0 && mode < 4 && (fp32 == 0 || fp32 == 1) && shape >= -1 && shape < 4, dohpERR_ARG,
+  DOHP_CHECK(op && mode >= 0 && mode < 4 && (fp32 == 0 || fp32 == 1) && shape >= -1 && shape < kShapes, dohpERR_ARG,
              "bad argument");
-  DOHP_CHECK(shape < 2 || op->scatter == 1, dohpERR_ARG, "the persistent shapes need the atomic scatter");
+  DOHP_CHECK((shape != 2 && shape != 3) || op->scatter == 1, dohpERR_ARG,
+             "the persistent shapes need the atomic scatter");
+  DOHP_CHECK(shape != 4 || (op->fs->n >= 4 && op->fs->n <= kMfmaMaxN), dohpERR_ARG,
+             "the matrix-core shape needs degree 3..7");
   op->shape[kApiMode[mode]][fp32] = shape;
   return 0;
 }

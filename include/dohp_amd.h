@@ -164,8 +164,11 @@ dErr dohpEllipGetStorage(dohpEllip op, int fp32, int *stored, float *ms);
  * 2 = Jacobian action recomputed from the state, 3 = application from stored quadrature data, linear
  * or Jacobian) and precision: -1 = not tuned yet, 0 = several elements per four-wavefront
  * workgroup, 1 = one-wavefront workgroups, 2 = persistent one-wavefront workgroups with the next
- * batch's loads in flight, 3 = persistent four-wavefront workgroups; ms[4] receives the measured
- * milliseconds of each. */
+ * batch's loads in flight, 3 = persistent four-wavefront workgroups, 4 = four-wavefront workgroups with
+ * the one-dimensional contractions as 16x16x4 MFMA products in the kernel's precision (degrees 3..7;
+ * no counterpart in the reference, whose TensorMult_Hex loops, src/jacobi/impls/tensor/inlinetmulthex.h,
+ * this shape replaces; timed by the tuner in FP32 only); ms[5] receives the measured milliseconds of each
+ * (0 for shapes not timed). */
 dErr dohpEllipGetTuning(dohpEllip op, int mode, int fp32, int *shape, float *ms);
 /* Fixes the launch shape of a mode and precision (-1 = tune again on the next application). */
 dErr dohpEllipSetShape(dohpEllip op, int mode, int fp32, int shape);

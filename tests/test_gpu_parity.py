@@ -278,12 +278,13 @@ def test_storage_choice_is_timed_and_kept(C):
 
 
 @pytest.mark.parametrize("storage", [0, 1])
-@pytest.mark.parametrize("p", [2, 4, 6, 9])
+@pytest.mark.parametrize("p", [2, 3, 4, 6, 7, 9])
 def test_every_launch_shape_matches_oracle(C, p, storage):
-    """Each launch shape the tuner chooses from (four-wavefront, one-wavefront, and their persistent
-    forms), pinned in turn, for the residual, the Jacobian action and the linear apply in FP64 (1e-12)
-    and FP32 (stated error), recomputed from the state (storage 0) and from stored quadrature data
-    (storage 1)."""
+    """Each launch shape the tuner chooses from (four-wavefront, one-wavefront, their persistent
+    forms and, for degrees 3..7, the four-wavefront shape with the contractions as MFMA products),
+    pinned in turn, for the residual, the Jacobian action and the linear apply in FP64 (1e-12) and
+    FP32 (stated error), recomputed from the state (storage 0) and from stored quadrature data
+    (storage 1).  The mesh's 18 elements leave empty slots in the last workgroup of every shape."""
     print(f"shapes p={p} storage={storage}", flush=True)
     M = (3, 2, 3)
     V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=17)
@@ -300,7 +301,7 @@ def test_every_launch_shape_matches_oracle(C, p, storage):
     op.set_storage(storage)
     lin.set_storage(storage)
     jmode, lmode = ("stored", "stored") if storage else ("jacobian", "linear")
-    for shape in range(4):
+    for shape in range(5 if 3 <= p <= 7 else 4):
         for mode in ("residual", jmode):
             op.set_shape(mode, 0, shape)
             op.set_shape(mode, 1, shape)
@@ -322,6 +323,48 @@ def test_every_launch_shape_matches_oracle(C, p, storage):
         assert max(errs) < TOL64, (shape, errs)
         assert max(e32) < 1e-4, (shape, e32)
         assert op.tuning(jmode, 0)[0] == shape and lin.tuning(lmode, 1)[0] == shape
+
+
+def test_matrix_core_shape_is_refused_outside_its_degrees(C):
+    """The MFMA shape exists for degrees 3..7: pinning it elsewhere is an argument error."""
+    for p, ok in ((2, False), (3, True), (7, True), (8, False)):
+        M = (2, 2, 2)
+        fs = C.FS(M, p, block_vertices(M, perturb=0.0, seed=1))
+        op = C.Ellip(fs, p=2.0)
+        if ok:
+            op.set_shape("linear", 0, 4)
+        else:
+            with pytest.raises(C.DohpError, match="matrix-core"):
+                op.set_shape("linear", 0, 4)
+
+
+@pytest.mark.parametrize("p", [5, 6])
+def test_matrix_core_shape_with_colored_scatter(C, p):
+    """The MFMA shape under the deterministic (coloured) scatter: equal to the atomic scatter's result
+    to rounding, and run to run bitwise."""
+    print(f"mfma colours p={p}", flush=True)
+    M = (4, 3, 5)
+    V = block_vertices(M, perturb=0.1, seed=3)
+    fs, S, E, _ = setup(C, M, p, V, p=2.0)
+    v = 2 * splitmix64_uniform(23, fs.nowned) - 1
+    vc = np.concatenate([v, np.zeros(fs.nclosure - fs.nowned)])
+    ref = E.jacobian_apply(np.zeros(fs.nclosure), vc)[:fs.nowned]
+    out = {}
+    for scatter in (0, 1):
+        op = C.Ellip(fs, p=2.0)
+        op.set_scatter(scatter)
+        op.set_storage(0)
+        op.set_shape("linear", 0, 4)
+        ys = []
+        for _ in range(2):
+            y = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+            op.mat_mult(dev(v), y)
+            ys.append(y.cpu().numpy())
+        assert rel(ys[0], ref) < TOL64
+        if scatter == 0:
+            assert np.array_equal(ys[0], ys[1])
+        out[scatter] = ys[0]
+    assert rel(out[0], out[1]) < 1e-14
 
 
 def test_atomic_and_colored_scatter_agree(C):

@@ -701,15 +701,20 @@ def main():
     tuning.update({f"jacobian_{'f32' if f else 'f64'}": op.tuning("jacobian", f) for f in (0, 1)})
     variants["tuning"] = tuning
     # the matrix-core shape (the one-dimensional contractions as 16x16x4 MFMA products in the kernel's
-    # precision), pinned, against the shape the tuner kept: time and relative difference of the results
+    # precision), pinned, against the shape the tuner kept: time and relative difference of the results (the
+    # residual first: the stored entries then form their quadrature data again, outside the timed calls)
     mfma = {}
     y_ref = torch.zeros_like(y)
-    op.set_storage(1)
     lin.set_storage(0)
-    for label, o, mode, fp32, apply in (
-            ("jv_f64_stored", op, "stored", 0, lambda o, out: o.mat_mult(v, out, stream)),
-            ("jv_f32_stored", op, "stored", 1, lambda o, out: o.mat_mult_f32(v32, out, stream)),
-            ("linear_f32", lin, "linear", 1, lambda o, out: o.mat_mult_f32(v32, out, stream))):
+    for label, o, mode, fp32, storage, apply in (
+            ("residual_f64", op, "residual", 0, 0, lambda o, out: o.function(u, out, stream)),
+            ("jv_f64_stored", op, "stored", 0, 1, lambda o, out: o.mat_mult(v, out, stream)),
+            ("jv_f32_stored", op, "stored", 1, 1, lambda o, out: o.mat_mult_f32(v32, out, stream)),
+            ("jv_f64_recomputed", op, "jacobian", 0, 0, lambda o, out: o.mat_mult(v, out, stream)),
+            ("jv_f32_recomputed", op, "jacobian", 1, 0, lambda o, out: o.mat_mult_f32(v32, out, stream)),
+            ("linear_f64", lin, "linear", 0, 0, lambda o, out: o.mat_mult(v, out, stream)),
+            ("linear_f32", lin, "linear", 1, 0, lambda o, out: o.mat_mult_f32(v32, out, stream))):
+        op.set_storage(storage)
         out = y if fp32 == 0 else y32
         ref = y_ref if fp32 == 0 else torch.zeros_like(y32)
         apply(o, ref)

@@ -586,7 +586,7 @@ def main():
 
     import torch
     import dohp_amd.capi as C
-    from dohp_amd.mesh import block_vertices, splitmix64_uniform
+    from dohp_amd.mesh import block_vertices, element_vertex_index, splitmix64_uniform
 
     dev = torch.device("cuda:0")
     Me, p = args.elems or 64, args.degree or 6
@@ -731,6 +731,29 @@ def main():
     lin.set_storage(2)
     del y_ref
     op.set_storage(2)
+    # the same J(u0) v on a space built from the block's element maps (dFSCreateFromElements): the kernels read
+    # the element-to-node and element-to-vertex maps (the 4 n^3 E bytes of the byte model) instead of
+    # evaluating the block numbering
+    fm = C.FS.from_elements(p, fs.element_nodes(), fs.nowned, fs.nclosure, V, element_vertex_index(M))
+    fm.dirichlet_commit(u0[fs.nowned:])
+    opm = C.Ellip(fm, p=1.5, eps=np.sqrt(0.005))
+    opm.set_scatter(args.scatter)
+    if stored64 in (0, 1):
+        opm.set_storage(stored64)
+    ym = torch.zeros_like(y)
+    opm.function(u, f, stream)
+    opm.mat_mult(v, ym, stream)
+    op.function(u, f, stream)
+    op.mat_mult(v, y, stream)
+    t_m = time_fn(torch, lambda: opm.mat_mult(v, ym, stream), args.steps)
+    torch.cuda.synchronize()
+    B_m = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True)
+    variants["jv_f64_element_maps"] = {
+        "us": round(t_m * 1e6, 1), "gdofs": round(fs.nowned / t_m / 1e9, 2),
+        "hbm_frac": round(B_m / t_m / 1e9 / HBM_PEAK_GBS, 4),
+        "rel_diff_vs_block": float(torch.linalg.norm(ym - y) / torch.linalg.norm(y))}
+    print(f"# jv_f64 on element maps {t_m * 1e6:.1f} us", file=sys.stderr, flush=True)
+    del opm, fm, ym
 
     B = alg_bytes(8, fs.nclosure, fs.nowned, n, fs.nelem, fs.nvert, True)
     t_k = t_kernel or t_step

@@ -54,6 +54,8 @@ _proto("dEFSApplyHexBatchedNoAlloc", ctypes.c_int, ctypes.c_int, _dp, ctypes.c_i
        ctypes.c_int, ctypes.c_int, _vp)
 _proto("dRuleComputePhysicalBatched", ctypes.c_int, _dp, ctypes.c_int, _vp, _vp, _vp, _vp)
 _proto("dFSCreateBlock", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(_vp))
+_proto("dFSCreateFromElements", ctypes.c_int, ctypes.c_int, _long, _long, _ip, _long, _dp, _ip,
+       ctypes.POINTER(_vp))
 _proto("dFSDestroy", ctypes.POINTER(_vp))
 _proto("dFSGetSizes", _vp, ctypes.POINTER(_long), ctypes.POINTER(_long), ctypes.POINTER(_long), _ip)
 _proto("dFSGetNodalCoordinatesClosure", _vp, _dp)
@@ -395,6 +397,29 @@ class FS:
         check(_lib.dFSGetSizes(h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
         self.nowned, self.nclosure, self.nexpanded, self.nelem = a.value, b.value, c.value, d.value
         self.nvert = v.shape[0] if v.ndim == 2 else v.size // 3
+
+    @classmethod
+    def from_elements(cls, degree, elem_nodes, nowned, nclosure, vcoords, elem_verts):
+        """Space on any conforming hexahedral mesh from its maps (dFSCreateFromElements): elem_nodes
+        [nelem][n^3] closure indices (owned first), elem_verts [nelem][8] vertex indices in tensor order,
+        vcoords [nvert][3]."""
+        self = cls.__new__(cls)
+        self.M = None
+        self.degree = int(degree)
+        E = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+        Vt = np.ascontiguousarray(elem_verts, dtype=np.int32)
+        v, vp = _np(vcoords)
+        nvert = v.shape[0] if v.ndim == 2 else v.size // 3
+        h = _vp()
+        check(_lib.dFSCreateFromElements(self.degree, int(E.shape[0]), int(nowned), int(nclosure),
+                                         E.ctypes.data_as(_ip), int(nvert), vp, Vt.ctypes.data_as(_ip),
+                                         ctypes.byref(h)))
+        self.h = h
+        a, b, c, d = _long(), _long(), _long(), ctypes.c_int()
+        check(_lib.dFSGetSizes(h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+        self.nowned, self.nclosure, self.nexpanded, self.nelem = a.value, b.value, c.value, d.value
+        self.nvert = nvert
+        return self
 
     def nodal_coordinates(self):
         X = np.zeros((self.nclosure, 3))

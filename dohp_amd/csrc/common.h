@@ -251,11 +251,15 @@ struct _p_dohpFS {
   std::vector<double> vcoords;       // [nvert][3]
   std::vector<int> elem_nodes;       // [nelem][n^3]
   std::vector<int> elem_verts;       // [nelem][8], tensor order
-  std::vector<int> color_off;        // 9 offsets into color_elems
+  int ncolor = 8;                    // element colours (elements of one colour share no node)
+  std::vector<int> color_off;        // ncolor + 1 offsets into color_elems
   std::vector<int> color_elems;      // elements grouped by (ex%2, ey%2, ez%2)
   std::vector<int> trav_elems;       // all elements in Morton (Z-curve) order
   std::vector<int> lat2clo;          // closure index of each lattice node (I, J, K), K fastest
   dohp::DevBuf<int> d_elem_nodes, d_elem_verts, d_color_elems, d_trav_elems, d_lat2clo;
   dohp::DevBuf<double> d_vcoords, d_dcache;
   dohp::DevBuf<float> d_dcache32;
+  // block spaces (dFSCreateBlock) have M > 0 and a lattice numbering the kernels evaluate; spaces built
+  // from element maps (dFSCreateFromElements) have M = 0 and are read through elem_nodes / elem_verts
+  bool is_block() const { return M[0] > 0; }
 };

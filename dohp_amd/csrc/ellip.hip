@@ -73,6 +73,7 @@ constexpr bool jac_state_first()
 template <int N, typename T, int MODE, int EPB, bool MF = false>
 __global__ void __launch_bounds__(MF ? 256 : wg_threads(N, EPB))
 __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <= 7 ? 3 : 1))) ellip_kernel(const int *__restrict__ elist, long ne, BoxMap bm,
+                                                    const int *__restrict__ enodes, const int *__restrict__ everts,
                                                     const double *__restrict__ vxyz, const double *__restrict__ kv,
                                                     const T *__restrict__ ustate, const T *__restrict__ xdir,
                                                     T *__restrict__ y, long nowned, int atomic,
@@ -96,17 +97,17 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
     double *dst = &VX[(l * 8 + vt) * 4];
     if (es < ne) {
       const int e = elist ? elist[es] : (int)es;
-      int ex, ey, ez;
-      elem_coords(bm, e, ex, ey, ez);
+      int ex = 0, ey = 0, ez = 0;
+      if (!enodes) elem_coords(bm, e, ex, ey, ez);
       if (MODE != MODE_JQD) {  // (the stored quadrature data carries the geometry)
-        const long vi = vertex_index(bm, ex, ey, ez, vt);
+        const long vi = enodes ? (long)everts[(long)e * 8 + vt] : vertex_index(bm, ex, ey, ez, vt);
         dst[0] = vxyz[3 * vi];
         dst[1] = vxyz[3 * vi + 1];
         dst[2] = vxyz[3 * vi + 2];
         dst[3] = ph.has_k ? kv[vi] : 1.0;
       }
-      if (vt == 0) {
-        EB[4 * l] = ex * bm.p;
+      if (vt == 0) {  // (a space from element maps keeps the element instead of its first lattice point)
+        EB[4 * l] = enodes ? e : ex * bm.p;
         EB[4 * l + 1] = ey * bm.p;
         EB[4 * l + 2] = ez * bm.p;
         EB[4 * l + 3] = 1;
@@ -139,11 +140,17 @@ __attribute__((amdgpu_waves_per_eu((MODE == MODE_FUN || MODE == MODE_QD) && N <=
       const int t = s * NT + tid;
       const bool in = t < EPB * N3;
       const int l = in ? t / N3 : 0;
-      int I = EB[4 * l], J = EB[4 * l + 1], K = EB[4 * l + 2];
-      node_lattice<N>(t - l * N3, I, J, K);
       const bool live = in && EB[4 * l + 3];
-      own[s] = live ? owned_index(bm, I, J, K) : -1;
-      lat[s] = live ? (I * bm.Ly + J) * bm.Lz + K : -1;
+      if (enodes) {  // element maps: closure index; the state is in closure order
+        const int c = live ? enodes[(long)EB[4 * l] * N3 + (t - l * N3)] : -1;
+        own[s] = c >= 0 && c < nowned ? c : -1;
+        lat[s] = c;
+      } else {
+        int I = EB[4 * l], J = EB[4 * l + 1], K = EB[4 * l + 2];
+        node_lattice<N>(t - l * N3, I, J, K);
+        own[s] = live ? owned_index(bm, I, J, K) : -1;
+        lat[s] = live ? (I * bm.Ly + J) * bm.Lz + K : -1;
+      }
     }
     T xv[NS], uv[NS];
 #pragma unroll
@@ -357,14 +364,15 @@ __global__ void zero_kernel(T *__restrict__ y, long n)
   if (i < n) y[i] = T(0);
 }
 
-// Linearization state in lattice order: owned values from u, Dirichlet values from the cache.
+// Linearization state in lattice order (closure order for a space from element maps, lat2clo = null): owned
+// values from u, Dirichlet values from the cache.
 __global__ void lattice_state_kernel(const double *__restrict__ u, const double *__restrict__ dcache,
                                      const int *__restrict__ lat2clo, long nowned, long nlat, double *__restrict__ st,
                                      float *__restrict__ st32)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nlat) return;
-  const long c = lat2clo[i];
+  const long c = lat2clo ? lat2clo[i] : i;
   const double v = c < nowned ? u[c] : dcache[c - nowned];
   st[i] = v;
   st32[i] = (float)v;
@@ -460,18 +468,22 @@ void launch_shape(const _p_dohpEllip *op, const T *ust, const T *x, T *y, hipStr
 {
   const dohpFS fs = op->fs;
   const dim3 block(MF ? 256 : wg_threads(N, EPB));
+  // a space from element maps is read through them (block spaces evaluate their numbering)
+  const int *enodes = fs->is_block() ? nullptr : fs->d_elem_nodes.ptr;
+  const int *everts = fs->is_block() ? nullptr : fs->d_elem_verts.ptr;
   if (op->scatter == 1) {
     const long ne = fs->nelem;
     hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB, MF>), dim3((ne + EPB - 1) / EPB), block, 0, st, nullptr, ne,
-                       box_map(fs), fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 1, gtab<T>(op),
-                       op->wphi.ptr, ph, op->nanflag.ptr);
+                       box_map(fs), enodes, everts, fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 1,
+                       gtab<T>(op), op->wphi.ptr, ph, op->nanflag.ptr);
   } else {
-    for (int col = 0; col < 8; col++) {
+    for (int col = 0; col < fs->ncolor; col++) {
       const long ne = fs->color_off[col + 1] - fs->color_off[col];
       if (!ne) continue;
       hipLaunchKernelGGL((ellip_kernel<N, T, MODE, EPB, MF>), dim3((ne + EPB - 1) / EPB), block, 0, st,
-                         fs->d_color_elems.ptr + fs->color_off[col], ne, box_map(fs), fs->d_vcoords.ptr,
-                         op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), op->wphi.ptr, ph, op->nanflag.ptr);
+                         fs->d_color_elems.ptr + fs->color_off[col], ne, box_map(fs), enodes, everts,
+                         fs->d_vcoords.ptr, op->kvert.ptr, ust, x, y, fs->nowned, 0, gtab<T>(op), op->wphi.ptr, ph,
+                         op->nanflag.ptr);
     }
   }
 }
@@ -527,7 +539,8 @@ dErr launch_n(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
 {
   const Phys ph = make_phys(op);
   int &shape = op->shape[MODE][sizeof(T) == 4];
-  if (op->scatter != 1 && (shape == 2 || shape == 3)) shape = -1;  // the persistent forms need the atomic scatter
+  // the persistent forms need the atomic scatter and a block space
+  if ((op->scatter != 1 || !op->fs->is_block()) && (shape == 2 || shape == 3)) shape = -1;
   if (shape < 0) {
     const long n = op->fs->nowned;
     DevBuf<T> scratch;
@@ -537,7 +550,7 @@ dErr launch_n(_p_dohpEllip *op, const T *ust, const T *x, T *y, hipStream_t st)
     DOHP_HIP(hipEventCreate(&e1));
     float best = 0;
     for (int v = 0; v < kShapes; v++) {
-      if ((v == 2 || v == 3) && op->scatter != 1) continue;
+      if ((v == 2 || v == 3) && (op->scatter != 1 || !op->fs->is_block())) continue;
       // (the matrix-core shape is timed in FP32 only: FP64 MFMA issues at the FP64 vector rate, so its padded
       // 16x16 tiles cannot win there; it stays available through dohpEllipSetShape)
       if (v == 4 && (!has_mfma_shape(N) || sizeof(T) != 4)) continue;
@@ -782,6 +795,8 @@ extern "C" dErr dohpEllipSetShape(dohpEllip op, int mode, int fp32, int shape)
              "bad argument");
   DOHP_CHECK((shape != 2 && shape != 3) || op->scatter == 1, dohpERR_ARG,
              "the persistent shapes need the atomic scatter");
+  DOHP_CHECK((shape != 2 && shape != 3) || op->fs->is_block(), dohpERR_ARG,
+             "the persistent shapes need a block space (dFSCreateBlock)");
   DOHP_CHECK(shape != 4 || (op->fs->n >= 4 && op->fs->n <= kMfmaMaxN), dohpERR_ARG,
              "the matrix-core shape needs degree 3..7");
   op->shape[kApiMode[mode]][fp32] = shape;

@@ -76,6 +76,7 @@ extern "C" dErr dFSCreateBlock(int M0, int M1, int M2, int degree, int dmask, co
         }
       }
   // Eight colours: elements of one colour share no node, so a colour can scatter without atomics.
+  fs->ncolor = 8;
   fs->color_off.assign(9, 0);
   fs->color_elems.resize(nel);
   for (int col = 0, k = 0; col < 8; col++) {
@@ -119,6 +120,91 @@ extern "C" dErr dFSCreateBlock(int M0, int M1, int M2, int degree, int dmask, co
     std::vector<double> zero(fs->nclosure - fs->nowned, 0.0);
     err = fs->d_dcache.upload(zero.data(), zero.size());
     std::vector<float> zf(fs->nclosure - fs->nowned, 0.0f);
+    if (!err) err = fs->d_dcache32.upload(zf.data(), zf.size());
+  }
+  if (err) {
+    delete fs;
+    return err;
+  }
+  *out = fs;
+  return 0;
+}
+
+// Continuous space from explicit element maps (the conforming case of dFSBuildSpace_Cont,
+// src/fs/impls/cont/cont.c:258-385, whose element assembly matrix E, cont.c:200-256, is then a gather by
+// index): elem_nodes[e][(a n + b) n + c] is the closure index of node (a, b, c) of element e (owned nodes
+// 0..nowned-1, Dirichlet nodes after them), elem_verts[e][(a 2 + b) 2 + c] its Q1 vertex in tensor order.
+// Elements are coloured greedily (lowest colour unused at any of their nodes) for the deterministic scatter.
+extern "C" dErr dFSCreateFromElements(int degree, int nelem, long nowned, long nclosure, const int *elem_nodes,
+                                      long nvert, const double *vcoords, const int *elem_verts, dohpFS *out)
+{
+  DOHP_CHECK(out, dohpERR_ARG, "null output");
+  *out = nullptr;
+  DOHP_CHECK(degree >= 1 && degree < kMaxN, dohpERR_ARG, "degree must be in [1,15]");
+  DOHP_CHECK(nelem > 0 && elem_nodes && elem_verts && vcoords && nvert > 0, dohpERR_ARG,
+             "elements, their node and vertex maps and the vertex coordinates are required");
+  DOHP_CHECK(nowned >= 0 && nowned <= nclosure && nclosure < (1L << 31) - 1, dohpERR_ARG,
+             "need 0 <= nowned <= nclosure < 2^31");
+  const int n = degree + 1, n3 = n * n * n;
+  DOHP_CHECK((long)nelem * n3 < (1L << 31) - 1, dohpERR_ARG, "too many expanded nodes for 32-bit offsets");
+  const long nexp = (long)nelem * n3;
+  std::vector<char> seen(nclosure, 0);
+  for (long i = 0; i < nexp; i++) {
+    DOHP_CHECK(elem_nodes[i] >= 0 && elem_nodes[i] < nclosure, dohpERR_ARG,
+               "element node index out of range at entry " + std::to_string(i));
+    seen[elem_nodes[i]] = 1;
+  }
+  for (long c = 0; c < nclosure; c++)
+    DOHP_CHECK(seen[c], dohpERR_ARG, "closure node " + std::to_string(c) + " belongs to no element");
+  for (long i = 0; i < (long)nelem * 8; i++)
+    DOHP_CHECK(elem_verts[i] >= 0 && elem_verts[i] < nvert, dohpERR_ARG,
+               "element vertex index out of range at entry " + std::to_string(i));
+  _p_dohpFS *fs = new _p_dohpFS;
+  fs->p = degree;
+  fs->n = n;
+  fs->nelem = nelem;
+  fs->nowned = nowned;
+  fs->nclosure = nclosure;
+  fs->dirichlet_mask = 0;
+  fs->nvert = nvert;
+  fs->vcoords.assign(vcoords, vcoords + 3 * nvert);
+  fs->elem_nodes.assign(elem_nodes, elem_nodes + nexp);
+  fs->elem_verts.assign(elem_verts, elem_verts + (long)nelem * 8);
+  {  // greedy colouring, at most 64 colours
+    std::vector<uint64_t> used(nclosure, 0);
+    std::vector<int> col(nelem);
+    int ncol = 0;
+    for (int e = 0; e < nelem; e++) {
+      uint64_t m = 0;
+      for (int k = 0; k < n3; k++) m |= used[elem_nodes[(long)e * n3 + k]];
+      if (!~m) {
+        delete fs;
+        DOHP_CHECK(false, dohpERR_ARG, "element colouring needs more than 64 colours");
+      }
+      const int c = __builtin_ctzll(~m);
+      col[e] = c;
+      ncol = std::max(ncol, c + 1);
+      for (int k = 0; k < n3; k++) used[elem_nodes[(long)e * n3 + k]] |= 1ULL << c;
+    }
+    fs->ncolor = ncol;
+    fs->color_off.assign(ncol + 1, 0);
+    for (int e = 0; e < nelem; e++) fs->color_off[col[e] + 1]++;
+    for (int c = 0; c < ncol; c++) fs->color_off[c + 1] += fs->color_off[c];
+    std::vector<int> pos(fs->color_off.begin(), fs->color_off.end() - 1);
+    fs->color_elems.resize(nelem);
+    for (int e = 0; e < nelem; e++) fs->color_elems[pos[col[e]]++] = e;
+  }
+  fs->trav_elems.resize(nelem);
+  for (int e = 0; e < nelem; e++) fs->trav_elems[e] = e;
+  dErr err = fs->d_elem_nodes.upload(fs->elem_nodes.data(), fs->elem_nodes.size());
+  if (!err) err = fs->d_trav_elems.upload(fs->trav_elems.data(), fs->trav_elems.size());
+  if (!err) err = fs->d_elem_verts.upload(fs->elem_verts.data(), fs->elem_verts.size());
+  if (!err) err = fs->d_color_elems.upload(fs->color_elems.data(), fs->color_elems.size());
+  if (!err) err = fs->d_vcoords.upload(fs->vcoords.data(), fs->vcoords.size());
+  if (!err && nclosure > nowned) {
+    std::vector<double> zero(nclosure - nowned, 0.0);
+    err = fs->d_dcache.upload(zero.data(), zero.size());
+    std::vector<float> zf(nclosure - nowned, 0.0f);
     if (!err) err = fs->d_dcache32.upload(zf.data(), zf.size());
   }
   if (err) {
@@ -246,7 +332,7 @@ extern "C" dErr dFSExpandedToGlobal(dohpFS fs, const double *x, double *g, int h
   hipStream_t st = (hipStream_t)stream;
   if (imode == dINSERT_VALUES)
     hipLaunchKernelGGL(fs_zero_kernel, dim3((fs->nowned + bs - 1) / bs), dim3(bs), 0, st, g, fs->nowned);
-  for (int col = 0; col < 8; col++) {
+  for (int col = 0; col < fs->ncolor; col++) {
     const int ne = fs->color_off[col + 1] - fs->color_off[col];
     if (!ne) continue;
     const long nt = (long)ne * n3;

@@ -616,6 +616,7 @@ extern "C" dErr dohpEllipAssembleQ1(dohpEllip op, const double *u, dohpMat *out,
   int q1mode = 0;
   DOHP_CALL(dohpEllipGetQ1Context_Private(op, &fs, &prm, &kvert, &state, &q1mode));
   DOHP_CHECK(fs->p <= kMaxP, dohpERR_ARG, "Q1 assembly supports degree <= 10");
+  DOHP_CHECK(fs->is_block(), dohpERR_ARG, "the embedded-Q1 matrix needs a block space (dFSCreateBlock)");
   const float *cscale = dohpEllipGetQ1CellScales_Private(op);
   hipStream_t st = (hipStream_t)stream;
   if (u) DOHP_CALL(dohpEllipFunctionState_Private(op, u, st));

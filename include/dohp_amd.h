@@ -91,6 +91,15 @@ typedef struct _p_dohpFS *dohpFS;
  * (set 100), 2 = top (200), 4 = bottom (300).  Closure numbering: owned nodes first, then Dirichlet. */
 dErr dFSCreateBlock(int M0, int M1, int M2, int degree, int dirichlet_mask, const double *vcoords_host,
                     dohpFS *fs);
+/* Continuous degree-p space on any conforming hexahedral mesh, from explicit maps (the conforming case
+ * of dFSBuildSpace_Cont, src/fs/impls/cont/cont.c:258-385, whose element assembly matrix E,
+ * cont.c:200-256, is then a gather by index).  elem_nodes_host: [nelem][n^3] closure index of node
+ * (a, b, c) at (a n + b) n + c, n = degree + 1 (owned nodes 0..nowned-1, Dirichlet nodes nowned..nclosure-1,
+ * every closure node in some element); elem_verts_host: [nelem][8] Q1 vertex of corner (a, b, c) at
+ * (a 2 + b) 2 + c; vcoords_host: [nvert][3].  The operator's kernels read the maps (the block form
+ * evaluates them); the embedded-Q1 matrix, its multigrid and the persistent launch shapes need a block. */
+dErr dFSCreateFromElements(int degree, int nelem, long nowned, long nclosure, const int *elem_nodes_host,
+                           long nvert, const double *vcoords_host, const int *elem_verts_host, dohpFS *fs);
 dErr dFSDestroy(dohpFS *fs);
 /* Sizes: owned (global) nodes, closure nodes, expanded nodes (nelem * n^3), elements. */
 dErr dFSGetSizes(dohpFS fs, long *nowned, long *nclosure, long *nexpanded, int *nelem);

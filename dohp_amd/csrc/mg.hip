@@ -160,8 +160,8 @@ void residual(const Box &bx, const T *H, const T *b, const T *x, T *r, hipStream
 // off-diagonals as h_ij = fp16(a_ij / (s_i s_j)) (|h_ij| <= 1 for a symmetric positive definite matrix) and, in FP32,
 // the diagonal d_i = a_ii + sum_j (a_ij - s_i s_j h_ij) over all 26 neighbours: the rounding of the off-diagonals
 // moves into the diagonal, so every row sum (and with it the near-null space of constants, on which the
-// preconditioner's quality rests) is that of the FP64 matrix.  A row then reads 26 + 8 bytes of coefficients
-// instead of 56.
+// preconditioner's quality rests) is that of the FP64 matrix.  A row then reads 28 + 8 bytes of coefficients
+// (the 13 off-diagonals padded to 7 pairs) instead of 56.
 __device__ __forceinline__ _Float16 coeff16(double a, float si, float sj)
 {
   return (_Float16)(float)(a / ((double)si * (double)sj));
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) half16_setup_kernel(Box bx, const double 
       h = coeff16(a, si, sj);
       comp += a - (double)si * (double)sj * (double)h;
     }
-    H[(s - 14) * n + i] = h;
+    H[((s - 14) >> 1) * 2 * n + 2 * i + ((s - 14) & 1)] = h;
     if (I - di >= 0 && I - di < bx.nx && J - dj >= 0 && J - dj < bx.ny && K - dk >= 0 && K - dk < bx.nz) {
       // the lower neighbour's entry is its upper one, rounded as that row rounds it
       const float sj = (float)sqrt(A[13 * n + i - off]);
@@ -198,6 +198,7 @@ __global__ void __launch_bounds__(256) half16_setup_kernel(Box bx, const double 
       comp += a - (double)sj * (double)si * (double)hl;
     }
   }
+  H[6 * 2 * n + 2 * i + 1] = 0;  // (the pad of the last pair)
   dg[i] = (float)(A[13 * n + i] + comp);
   sc[i] = si;
 }
@@ -229,20 +230,32 @@ __device__ __forceinline__ void stage_scaled(const Box &bx, const float *__restr
   __syncthreads();
 }
 
-// sum_j h_ij s_j x_j over the 26 neighbours of row i (the thread's row of the block), from the staged tile.  A
-// neighbour outside the box has a zero tile entry; its coefficient is read at a clamped index.
+// sum_j h_ij s_j x_j over the 26 neighbours of row i (the thread's row of the block), from the staged tile.  The
+// coefficients are stored as pairs of directions per row ([7][n] dwords, directions 14 + 2j and 15 + 2j, the last
+// pair padded): a row reads its 13 upper ones with 7 dword loads, and each lower one (the upper one of the lower
+// neighbour's row) with one, instead of 26 two-byte loads.  A neighbour outside the box has a zero tile entry; its
+// coefficient is read at a clamped index.
+__device__ __forceinline__ float half16_value(unsigned pair, int hi)
+{
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)(hi ? pair >> 16 : pair & 0xffffu));
+}
 __device__ __forceinline__ float half16_tile_row(const Box &bx, const _Float16 *__restrict__ H, const float *tile,
                                                  long i)
 {
   const long n = bx.n(), sy = bx.nz, sx = (long)bx.ny * bx.nz;
+  const unsigned *__restrict__ H2 = (const unsigned *)H;
   const int tj = threadIdx.y + 1, tk = threadIdx.x + 1;
+  unsigned up[7];
+#pragma unroll
+  for (int j = 0; j < 7; j++) up[j] = H2[j * n + i];
   float acc = 0;
 #pragma unroll
   for (int s = 14; s < 27; s++) {
     const int di = s / 9 - 1, dj = (s / 3) % 3 - 1, dk = s % 3 - 1;
     const long off = di * sx + dj * sy + dk;
     const long il = i - off >= 0 && i - off < n ? i - off : i;
-    const float hu = (float)H[(s - 14) * n + i], hl = (float)H[(s - 14) * n + il];
+    const float hu = half16_value(up[(s - 14) >> 1], (s - 14) & 1);
+    const float hl = half16_value(H2[((s - 14) >> 1) * n + il], (s - 14) & 1);
     acc += hu * tile[((1 + di) * kTY + tj + dj) * kTX + tk + dk];
     acc += hl * tile[((1 - di) * kTY + tj - dj) * kTX + tk - dk];
   }
@@ -623,7 +636,7 @@ struct MGLevel {
   const double *A64 = nullptr;  // full 27-point stencil (finest level: the Q1 matrix's own)
   DevBuf<double> own64, H64, dinv64, b64, x64, r64, d64;  // H: symmetric half stencil [14][n]
   DevBuf<float> H32, dinv32, b32, x32, r32, d32;
-  DevBuf<_Float16> H16;        // FP16 upper off-diagonals [13][n] (see half16_setup_kernel)
+  DevBuf<_Float16> H16;        // FP16 upper off-diagonals, pairs of directions per row [7][n][2] (half16_tile_row)
   DevBuf<float> dg16, sc16, t32;  // their diagonal and row scales; the second iterate buffer of the FP16 V-cycle
   double lmax = 0;
 };
@@ -934,7 +947,7 @@ extern "C" dErr dohpMGCreate(dohpMat q1, int smooth_degree, int max_coarse, dohp
     if (!err && l > 0) err = lv.x64.alloc(n);
     if (!err && l > 0) err = lv.b32.alloc(n);
     if (!err && l > 0) err = lv.x32.alloc(n);
-    if (!err && l + 1 < mg->L.size()) err = lv.H16.alloc(13 * n);
+    if (!err && l + 1 < mg->L.size()) err = lv.H16.alloc(14 * n);
     if (!err && l + 1 < mg->L.size()) err = lv.dg16.alloc(n);
     if (!err && l + 1 < mg->L.size()) err = lv.sc16.alloc(n);
     if (!err && l + 1 < mg->L.size()) err = lv.t32.alloc(n);

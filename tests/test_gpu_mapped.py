@@ -102,7 +102,8 @@ def test_block_maps_give_the_block_space(C, p):
 
 def test_any_numbering_gives_the_same_operator(C):
     """Random renumbering of elements, owned nodes, Dirichlet nodes and vertices, with a per-vertex
-    coefficient: the operator is the block space's, renumbered."""
+    coefficient, the Bratu term and the exact solution's forcing term: the operator is the block space's,
+    renumbered."""
     M, p = (4, 3, 2), 5
     V = block_vertices(M, perturb=0.15, seed=8)
     fs = C.FS(M, p, V)
@@ -127,7 +128,7 @@ def test_any_numbering_gives_the_same_operator(C):
     fm.dirichlet_commit(uc[inv][fs.nowned:])
     out = {}
     for name, space, kvert, perm in (("block", fs, kv, None), ("maps", fm, kv[pv], inv[:fs.nowned])):
-        op = C.Ellip(space, p=1.7, eps=0.1, lam=0.3, kvert=kvert)
+        op = C.Ellip(space, p=1.7, eps=0.1, lam=0.3, exact=0, kvert=kvert)  # (with the forcing term)
         lin = C.Ellip(space, p=2.0, kvert=kvert)
         uu = uc[:fs.nowned] if perm is None else uc[perm]
         vv = v if perm is None else v[perm]

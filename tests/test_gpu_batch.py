@@ -104,6 +104,31 @@ def test_batch_fp64_and_fp32_match_oracle(C):
             assert tile in (0, 1) and ms[tile] > 0 and ms[2] < 0
 
 
+def test_batch_workload_generator_matches_oracle(C):
+    """The bench's mixed-order workload (mesh.mixed_order_batch) at 8^3 mixed elements (degrees 2..10) and 8
+    Poisson problems (n^3 meshes, n in 4..16, degrees 2..8): every segment against the oracle."""
+    from dohp_amd.mesh import mixed_order_batch
+    print("batch workload generator 8^3 + 8 problems", flush=True)
+    deg, vx, probs = mixed_order_batch(8, 8)
+    spaces = [(M, p, V, None) for (M, p, V) in probs]
+    B, keep, segs = build(C, spaces, [(deg, vx, None)])
+    n = B.size
+    x = 2 * splitmix64_uniform(12, n) - 1
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    B.mult(dev(x), y)
+    y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    B.mult_f32(dev(x, torch.float32), y32)
+    yh, y32h = y.cpu().numpy(), y32.cpu().numpy()
+    worst64 = worst32 = 0.0
+    for kind, s, data in segs:
+        off, ln = B.segment(s)
+        ref = reference(kind, data, x[off:off + ln])
+        e64, e32 = rel(yh[off:off + ln], ref), rel(y32h[off:off + ln], ref)
+        worst64, worst32 = max(worst64, e64), max(worst32, e32)
+        assert e64 < TOL64 and e32 < TOL32, (kind, s, e64, e32)
+    print(f"{len(segs)} segments, {n} unknowns: fp64 {worst64:.1e}, fp32 {worst32:.1e}")
+
+
 def test_batch_repeat_and_layout(C):
     print("batch repeat", flush=True)
     spaces, elems = case(33)

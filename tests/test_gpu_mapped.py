@@ -100,6 +100,39 @@ def test_block_maps_give_the_block_space(C, p):
                 assert rel(m[i], b[i]) < (1e-6 if i == 3 else 1e-14), (shape, storage, i)
 
 
+@pytest.mark.parametrize("p", [1, 2, 3, 5, 7, 8, 10])
+def test_block_maps_every_degree(C, p):
+    """Every degree the fused kernels compile: the residual, J v (stored and recomputed) and the linear apply on
+    a block space's own maps equal the block space's (the tuner's shapes)."""
+    print(f"maps every degree p={p}", flush=True)
+    M = (2, 3, 2)
+    V = block_vertices(M, perturb=0.1, seed=40 + p)
+    fs = C.FS(M, p, V)
+    fm = C.FS.from_elements(p, fs.element_nodes(), fs.nowned, fs.nclosure, V, block_vertex_map(M))
+    X = fs.nodal_coordinates()
+    uc = np.sin(2 * X).sum(axis=1)
+    v = 2 * splitmix64_uniform(50 + p, fs.nowned) - 1
+    res = {}
+    for name, space in (("block", fs), ("maps", fm)):
+        space.dirichlet_commit(uc[fs.nowned:])
+        op = C.Ellip(space, p=1.5, eps=0.1)
+        lin = C.Ellip(space, p=2.0)
+        out = []
+        for storage in (0, 1):
+            op.set_storage(storage)
+            f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+            op.function(dev(uc[:fs.nowned]), f)
+            y = torch.zeros_like(f)
+            op.mat_mult(dev(v), y)
+            yl = torch.zeros_like(f)
+            lin.mat_mult(dev(v), yl)
+            out += [f.cpu().numpy(), y.cpu().numpy(), yl.cpu().numpy()]
+        res[name] = out
+    errs = [rel(m, b) for m, b in zip(res["maps"], res["block"])]
+    print("maps vs block", ["%.1e" % e for e in errs])
+    assert max(errs) < 1e-14, errs
+
+
 def test_any_numbering_gives_the_same_operator(C):
     """Random renumbering of elements, owned nodes, Dirichlet nodes and vertices, with a per-vertex
     coefficient, the Bratu term and the exact solution's forcing term: the operator is the block space's,

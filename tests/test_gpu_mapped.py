@@ -176,6 +176,52 @@ def test_any_numbering_gives_the_same_operator(C):
         assert max(errs[:3]) < 1e-13 and errs[3] < 1e-6, (storage, errs)
 
 
+def test_element_orientations_do_not_change_the_operator(C):
+    """Elements seen through a random rotation of the cube (axes permuted and reflected with determinant +1, the node
+    and the vertex tensors alike: a reflection would turn the element inside out, which is an error), as neighbouring
+    elements of an unstructured mesh are: the Gauss rule is symmetric, so the operator is the block space's to
+    rounding."""
+    M, p = (3, 3, 3), 4
+    V = block_vertices(M, perturb=0.12, seed=21)
+    fs = C.FS(M, p, V)
+    n = p + 1
+    rng = np.random.default_rng(17)
+    E = fs.element_nodes().reshape(-1, n, n, n)
+    EV = block_vertex_map(M).reshape(-1, 2, 2, 2)
+    En, EVn = E.copy(), EV.copy()
+    turned = 0
+    for e in range(fs.nelem):
+        if rng.random() < 0.3:
+            continue
+        axes, flips = rng.permutation(3), rng.random(3) < 0.5
+        odd_perm = sum(axes[i] > axes[j] for i in range(3) for j in range(i + 1, 3)) % 2
+        if (odd_perm + flips.sum()) % 2:  # keep the orientation: one more reflection
+            flips[2] = not flips[2]
+        def orient(T):
+            T = np.transpose(T, axes)
+            for d in range(3):
+                if flips[d]:
+                    T = np.flip(T, axis=d)
+            return T
+        En[e], EVn[e] = orient(E[e]), orient(EV[e])
+        turned += 1
+    fm = C.FS.from_elements(p, En.reshape(fs.nelem, -1), fs.nowned, fs.nclosure, V, EVn.reshape(fs.nelem, 8))
+    X = fs.nodal_coordinates()
+    np.testing.assert_allclose(fm.nodal_coordinates(), X, rtol=0, atol=1e-14)
+    uc = np.sin(2 * X).sum(axis=1)
+    v = 2 * splitmix64_uniform(61, fs.nowned) - 1
+    res = {}
+    for name, space in (("block", fs), ("turned", fm)):
+        space.dirichlet_commit(uc[fs.nowned:])
+        op = C.Ellip(space, p=1.5, eps=0.1, exact=0)
+        lin = C.Ellip(space, p=2.0)
+        res[name] = applies(C, space, op, lin, uc[:fs.nowned], v, (0,))[0]
+    for storage in (0, 1):
+        errs = [rel(res["turned"][storage][i], res["block"][storage][i]) for i in range(4)]
+        print(f"{turned} of {fs.nelem} elements turned, storage {storage}: {['%.1e' % e for e in errs]}")
+        assert max(errs[:3]) < 1e-13 and errs[3] < 1e-6, (storage, errs)
+
+
 def test_gather_scatter_and_colours_on_maps(C):
     """dFSGlobalToExpanded / dFSExpandedToGlobal through renumbered maps (greedy colours), and the
     operator's deterministic colour scatter against its atomic one."""

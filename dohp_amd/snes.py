@@ -66,8 +66,9 @@ def gmres(A, M, b, rtol=1e-5, atol=1e-50, dtol=1e5, restart=30, max_it=10000):
             prec(t, w)
             h = V[:j + 1] @ w  # classical Gram-Schmidt: every projection against the same vector
             w -= V[:j + 1].T @ h
-            hn = _norm(w)
-            col = np.concatenate([h.cpu().numpy(), [hn]])
+            # the Hessenberg column and the new vector's norm come back to the host in one read
+            col = torch.cat([h, torch.linalg.vector_norm(w).reshape(1)]).cpu().numpy().astype(np.float64)
+            hn = float(col[j + 1])
             for i in range(j):  # previous rotations
                 a, c = col[i], col[i + 1]
                 col[i], col[i + 1] = cs[i] * a + sn[i] * c, -sn[i] * a + cs[i] * c

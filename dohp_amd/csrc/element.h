@@ -202,11 +202,18 @@ __device__ __forceinline__ void eo_apply(const T *__restrict__ tab, int op, cons
 
 // Element tiles [i][j][k] (k fastest, the reference's node order).  A thread (a, b) of an
 // element owns one line per phase: a z line (i, j) = (a, b), a y line (i, k) = (a, b) or an
-// x line (j, k) = (a, b).
+// x line (j, k) = (a, b).  No register value crosses from one phase to the next, so any one-to-one
+// assignment of lines to threads within a phase is valid.  At n = 7 the y lines are dealt out as
+// (i, k) = (b, (7 - a) mod 7): with (a, b) the lanes of a wavefront start at a * 49 + b, which
+// puts several lanes of each 16 on the same LDS bank pair (8-byte slots); the permuted starts
+// spread them (2.9 -> 2.0 accesses per bank per pass on 32 banks, counted on the host; the FP64
+// residual and recomputing kernels at degree 6 run 1.3-1.9 % faster).  At the other orders no
+// permutation of this kind measured faster, and (a, b) is kept.
 enum { DIR_X = 0, DIR_Y = 1, DIR_Z = 2 };
 template <int N, int DIR>
 __device__ __forceinline__ int line_at(int a, int b, int s)
 {
+  if constexpr (DIR == DIR_Y && N == 7) return (b * N + s) * N + (a == 0 ? 0 : N - a);
   return DIR == DIR_Z ? (a * N + b) * N + s : DIR == DIR_Y ? (a * N + s) * N + b : (s * N + a) * N + b;
 }
 template <int N, int DIR, typename T>

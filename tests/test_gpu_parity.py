@@ -387,6 +387,37 @@ def test_atomic_and_colored_scatter_agree(C):
     assert torch.equal(y0, y2)
 
 
+@pytest.mark.parametrize("storage", [0, 1])
+def test_mat_mult_add_accumulates(C, storage):
+    """MatMultAdd (the shell's MATOP_MULT_ADD): y0 + J(u) v against the oracle in every launch shape, from
+    stored quadrature data and recomputed, on columns of six elements along z (five-element workgroups fold
+    the faces they share); refused under the colour scatter, which does not accumulate."""
+    M, p = (2, 2, 6), 6
+    print(f"MatMultAdd storage={storage}", flush=True)
+    V = block_vertices(M, box=(0, 1, 0, 1, 0, 1), perturb=0.1, seed=31)
+    fs, S, E, op = setup(C, M, p, V, p=1.5, eps=np.sqrt(0.005))
+    uc = sine_state(S.nodal_coordinates(), 5)
+    fs.dirichlet_commit(uc[fs.nowned:])
+    f = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+    op.function(dev(uc[:fs.nowned]), f)
+    op.set_storage(storage)
+    v = 2 * splitmix64_uniform(17, fs.nowned) - 1
+    y0 = 2 * splitmix64_uniform(19, fs.nowned) - 1
+    ref = y0 + E.jacobian_apply(uc, np.concatenate([v, np.zeros(fs.nclosure - fs.nowned)]))[:fs.nowned]
+    mode = "stored" if storage else "jacobian"
+    for shape in range(5):
+        op.set_shape(mode, 0, shape)
+        y = dev(y0)
+        op.mat_mult_add(dev(v), y)
+        assert rel(y.cpu().numpy(), ref) < TOL64, shape
+        y = torch.zeros(fs.nowned, dtype=torch.float64, device="cuda")
+        op.mat_mult(dev(v), y)
+        assert rel(y.cpu().numpy() + y0, ref) < TOL64, shape
+    op.set_scatter(0)
+    with pytest.raises(C.DohpError):
+        op.mat_mult_add(dev(v), dev(y0))
+
+
 @pytest.mark.parametrize("p", list(range(1, 11)))
 def test_poisson_apply_all_orders(C, p):
     M = (2, 2, 2)

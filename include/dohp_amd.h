@@ -144,6 +144,10 @@ dErr dohpEllipFunction(dohpEllip op, const double *u_dev, double *f_dev, void *s
  * MATOP_MULT body.  FP64 and FP32 variants; the FP32 one keeps geometry in FP64. */
 dErr dohpEllipShellMatMult(dohpEllip op, const double *x_dev, double *y_dev, void *stream);
 dErr dohpEllipShellMatMultF32(dohpEllip op, const float *x_dev, float *y_dev, void *stream);
+/* y += J(u) x: MATOP_MULT_ADD with v2 = v3 for the same MATSHELL (ellip.c:442-472 sets MATOP_MULT only,
+ * so PETSc forms MatMultAdd there as MatMult into a work vector plus VecAXPY; this adds in the scatter).
+ * Needs the atomic scatter (dohpERR_ARG under the colour scatter); checked like MatMult. */
+dErr dohpEllipShellMatMultAdd(dohpEllip op, const double *x_dev, double *y_dev, void *stream);
 /* 1 (default): the residual and MatMult entry points wait for their stream and report non-finite values
  * (dohpERR_NAN); 0: they return at once, unchecked (asynchronous use; the library's solvers apply the
  * operator unchecked and test the FP64 residual norm instead, returning dohpERR_NAN if it is not finite). */

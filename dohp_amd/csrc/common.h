@@ -254,9 +254,8 @@ struct _p_dohpFS {
   int ncolor = 8;                    // element colours (elements of one colour share no node)
   std::vector<int> color_off;        // ncolor + 1 offsets into color_elems
   std::vector<int> color_elems;      // elements grouped by (ex%2, ey%2, ez%2)
-  std::vector<int> trav_elems;       // all elements in Morton (Z-curve) order
   std::vector<int> lat2clo;          // closure index of each lattice node (I, J, K), K fastest
-  dohp::DevBuf<int> d_elem_nodes, d_elem_verts, d_color_elems, d_trav_elems, d_lat2clo;
+  dohp::DevBuf<int> d_elem_nodes, d_elem_verts, d_color_elems, d_lat2clo;
   dohp::DevBuf<double> d_vcoords, d_dcache;
   dohp::DevBuf<float> d_dcache32;
   // block spaces (dFSCreateBlock) have M > 0 and a lattice numbering the kernels evaluate; spaces built

@@ -86,33 +86,9 @@ extern "C" dErr dFSCreateBlock(int M0, int M1, int M2, int degree, int dmask, co
         for (long e2 = col & 1; e2 < M2; e2 += 2) fs->color_elems[k++] = (int)((e0 * M1 + e1) * M2 + e2);
     fs->color_off[col + 1] = k;
   }
-  // Z-curve traversal: elements that share nodes are processed close together in time, so
-  // the gather and the atomic scatter reuse cache lines before they are evicted.
-  {
-    auto spread = [](uint64_t v) {
-      uint64_t x = v & 0x1fffff;
-      x = (x | x << 32) & 0x1f00000000ffffULL;
-      x = (x | x << 16) & 0x1f0000ff0000ffULL;
-      x = (x | x << 8) & 0x100f00f00f00f00fULL;
-      x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
-      x = (x | x << 2) & 0x1249249249249249ULL;
-      return x;
-    };
-    std::vector<std::pair<uint64_t, int>> key(nel);
-    for (long e0 = 0; e0 < M0; e0++)
-      for (long e1 = 0; e1 < M1; e1++)
-        for (long e2 = 0; e2 < M2; e2++) {
-          const long e = (e0 * M1 + e1) * M2 + e2;
-          key[e] = {spread(e0) << 2 | spread(e1) << 1 | spread(e2), (int)e};
-        }
-    std::sort(key.begin(), key.end());
-    fs->trav_elems.resize(nel);
-    for (long k = 0; k < nel; k++) fs->trav_elems[k] = key[k].second;
-  }
   fs->lat2clo = std::move(l2c);
   dErr err = fs->d_elem_nodes.upload(fs->elem_nodes.data(), fs->elem_nodes.size());
   if (!err) err = fs->d_lat2clo.upload(fs->lat2clo.data(), fs->lat2clo.size());
-  if (!err) err = fs->d_trav_elems.upload(fs->trav_elems.data(), fs->trav_elems.size());
   if (!err) err = fs->d_elem_verts.upload(fs->elem_verts.data(), fs->elem_verts.size());
   if (!err) err = fs->d_color_elems.upload(fs->color_elems.data(), fs->color_elems.size());
   if (!err) err = fs->d_vcoords.upload(fs->vcoords.data(), fs->vcoords.size());
@@ -194,10 +170,7 @@ extern "C" dErr dFSCreateFromElements(int degree, int nelem, long nowned, long n
     fs->color_elems.resize(nelem);
     for (int e = 0; e < nelem; e++) fs->color_elems[pos[col[e]]++] = e;
   }
-  fs->trav_elems.resize(nelem);
-  for (int e = 0; e < nelem; e++) fs->trav_elems[e] = e;
   dErr err = fs->d_elem_nodes.upload(fs->elem_nodes.data(), fs->elem_nodes.size());
-  if (!err) err = fs->d_trav_elems.upload(fs->trav_elems.data(), fs->trav_elems.size());
   if (!err) err = fs->d_elem_verts.upload(fs->elem_verts.data(), fs->elem_verts.size());
   if (!err) err = fs->d_color_elems.upload(fs->color_elems.data(), fs->color_elems.size());
   if (!err) err = fs->d_vcoords.upload(fs->vcoords.data(), fs->vcoords.size());
